@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: syn events/s and x-realtime of the 1M-neuron / 100M-synapse Izhikevich + pair-STDP
+network on one MI355X, with the propagation/update kernel roofline and a CPU baseline.
+
+A step is one 1 ms simulation step of the whole network (neuron update, propagation with delays,
+STDP).  `value` = synaptic events (sum of out-degrees of spiking rows) per wall second over the
+K timed steps, inputs (network, state) resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def neuron_bytes(cfg, n, spikes):
+    """Algorithmic bytes of one update launch: per-neuron state read+write, ring slot read+clear,
+    plus per-spike state (jitter c/d, count, last step, pre trace record, spike list)."""
+    from neuralsim_amd import abi
+    if cfg.model == abi.MODEL_IZHIKEVICH:
+        per_n, per_s = 8 + 8 + 16, 8 + 8 + 4 + 16 + 4
+    elif cfg.model == abi.MODEL_LIF:
+        per_n, per_s = 8 + 8 + 8 + 8 + 32, 8 + 4 + 4
+    else:
+        per_n, per_s = 4 * 8 * 2 + 32, 8 + 4 + 4
+    return per_n * n + per_s * spikes
+
+
+def cpu_baseline(spec, gpu_events_per_step, steps):
+    """FP64 restatement of the reference's step (per-synapse STDP traces, serial O(M) synapse
+    passes as engine.rs:397-537) timed on one host core over a bounded window of the same
+    network; converted to the metric's unit with the device run's mean events per step."""
+    from oracle import oracle
+    t0 = time.perf_counter()
+    src, dst, w, d = oracle.generate_list(spec)
+    rp, col, perm = oracle.finalize(spec.n, src, dst)
+    w, d = w[perm], d[perm]
+    del src, dst, perm
+    ref = oracle.RefNet(spec.cfg, rp, col, w, d)
+    del rp, col, w, d
+    setup = time.perf_counter() - t0
+    log(f"cpu baseline: network ready in {setup:.1f}s, timing {steps} steps")
+    t0 = time.perf_counter()
+    ref.run(steps, cap=1 << 24)
+    dt = time.perf_counter() - t0
+    ref.close()
+    s_per_step = dt / steps
+    return {
+        "value": gpu_events_per_step / s_per_step,
+        "unit": "syn events/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{steps} steps (from t=0) of the same 1M/100M network in the oracle's FP64 "
+                   f"restatement of SimulationEngine::step with per-synapse STDP traces, "
+                   f"{s_per_step * 1e3:.0f} ms/step on 1 core; events/s = device mean events "
+                   f"per step / CPU seconds per step"),
+        "ms_per_step": s_per_step * 1e3,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10_000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--neurons", type=int, default=1_000_000)
+    ap.add_argument("--graph", type=int, default=0, help="steps per captured hipGraph (0: off)")
+    ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import neuralsim_amd as ns
+
+    spec = ns.configs.izh_stdp_network(n=args.neurons, steps=args.steps)
+    t0 = time.perf_counter()
+    eng = ns.SimulationEngine.from_spec(spec)
+    build_s = time.perf_counter() - t0
+    log(f"bench: {spec.name} n={spec.n} m={spec.m} built on device in {build_s:.2f}s")
+    if args.graph:
+        eng.set_graph_steps(args.graph)
+    eng.step(args.warmup)
+    log(f"bench: warmup {args.warmup} steps done")
+    prof = not args.no_profile and not args.graph
+    eng.set_profiling(prof)
+    t0 = time.perf_counter()
+    st = eng.step(args.steps)
+    wall = time.perf_counter() - t0
+    log(f"bench: timed {args.steps} steps in {wall:.3f}s (device {st.wall_ms / 1e3:.3f}s)")
+    sim_s = args.steps * spec.cfg.dt_ms / 1000.0
+    events, spikes = st.events, st.spikes
+    value = events / wall
+    out = {
+        "metric": "syn events/s & x-realtime @1M/100M Izh+STDP; HH surrogate vs plain speedup",
+        "value": value,
+        "unit": "syn events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox-generated network and drive)",
+        "config": {
+            "workload": "izhikevich_pair_stdp_1M_100M",
+            "neurons": spec.n, "synapses": spec.m, "dt_ms": spec.cfg.dt_ms,
+            "v_substeps": spec.cfg.v_substeps, "delays_ms": [1, 20], "sim_ms": sim_s * 1e3,
+            "accumulate": "int64 fixed point 2^-32", "graph_steps": args.graph,
+        },
+        "x_realtime": sim_s / wall,
+        "spikes": spikes,
+        "events": events,
+        "mean_rate_hz": spikes / spec.n / sim_s,
+        "build_s": build_s,
+    }
+    if prof:
+        kinds = {0: "k_update", 1: "k_propagate", 2: "k_ltd"}
+        times = {k: eng.kernel_time(k) for k in kinds}
+        upd_b = neuron_bytes(spec.cfg, spec.n, spikes / args.steps)
+        prop_b = (17.0 * events + 16.0 * spikes) / args.steps
+        kt = {kinds[k]: {"avg_us": (ms / max(n, 1)) * 1e3, "launches": n}
+              for k, (ms, n) in times.items()}
+        kt["k_update"]["alg_bytes_per_launch"] = upd_b
+        kt["k_propagate"]["alg_bytes_per_launch"] = prop_b
+        for k in ("k_update", "k_propagate"):
+            a = kt[k]["alg_bytes_per_launch"] / (kt[k]["avg_us"] * 1e-6) / 1e9
+            kt[k]["achieved_GBs"] = a
+        dom = max(("k_update", "k_propagate"), key=lambda k: kt[k]["avg_us"])
+        out["roofline"] = {
+            "kernel": dom, "bound": "hbm", "achieved": kt[dom]["achieved_GBs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kt[dom]["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": None,
+        }
+        out["kernels"] = kt
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(spec, events / args.steps, args.cpu_steps)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,265 @@
+/*
+ * nsx.h -- C ABI of the MI355X (gfx950) NeuralSim simulation core.
+ *
+ * The library replaces NeuralSim's wgpu `GpuBackend` (src/simulation/gpu_backend.rs:47-1037),
+ * the slot `SimulationEngine::step()` dispatches into when a GPU backend is present
+ * (src/simulation/engine.rs:254-258).  Two groups of entry points are exported:
+ *
+ *   1. nsx_backend_*  -- one-to-one replacements of GpuBackend's methods, same argument meaning,
+ *      same per-call host round trip.  A Rust `extern "C"` block binds these where
+ *      gpu_backend.rs is compiled today (binding shown in INTEGRATION.md).
+ *   2. nsx_*          -- the device-resident engine: the network, neuron state, delay rings,
+ *      plasticity traces and drives live in HBM; nsx_step() advances many steps per call and
+ *      only statistics cross PCIe.  This is what SimulationEngine::step()/simulate_ms()
+ *      (engine.rs:254-797) become when the whole step runs on the GPU.
+ *
+ * Conventions (all entry points):
+ *   - Every call returns an nsx_status; nothing aborts or throws across the ABI.  The text of the
+ *     last error of a context is available from nsx_last_error(ctx); errors raised before a
+ *     context exists are available from nsx_last_error(NULL) (thread-local).
+ *   - The caller owns every host buffer.  The library copies, never retains, host pointers.
+ *   - One context is driven by one host thread at a time (GpuBackend methods take &mut self).
+ *     Separate contexts are independent.
+ *   - Calls are synchronous with respect to host-visible outputs.  Streams/graphs are internal.
+ *   - Neuron indices [0, n_exc) are excitatory, [n_exc, n_neurons) inhibitory.
+ */
+#ifndef NSX_H
+#define NSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSX_ABI_VERSION 1u
+
+/* GpuError {NoAdapter, DeviceRequest} (gpu_backend.rs:24-39) widened to every failure the
+ * device path can report; maps onto NeuralSimError::Gpu (src/error.rs:44-46). */
+typedef enum nsx_status {
+  NSX_OK = 0,
+  NSX_E_NODEVICE = -1, /* no usable gfx950 device (GpuError::NoAdapter)            */
+  NSX_E_OOM = -2,      /* device allocation failed                                 */
+  NSX_E_INVALID = -3,  /* bad argument / configuration                             */
+  NSX_E_BADCSR = -4,   /* row_ptr not monotone, column or delay out of range        */
+  NSX_E_HIP = -5,      /* any other HIP runtime error (GpuError::DeviceRequest)     */
+  NSX_E_STATE = -6,    /* call out of order (e.g. step before a network is loaded)  */
+  NSX_E_OVERFLOW = -7  /* a bounded device buffer (spike list, raster) overflowed   */
+} nsx_status;
+
+/* NeuronModelParams variants (src/neuron/mod.rs:72-94). */
+typedef enum nsx_model {
+  NSX_MODEL_LIF = 0,
+  NSX_MODEL_IZHIKEVICH = 1,
+  NSX_MODEL_HH = 2,
+  NSX_MODEL_HH_SURROGATE = 3 /* HH whose gating step is a learned MLP (no reference twin) */
+} nsx_model;
+
+typedef enum nsx_drive_kind {
+  NSX_DRIVE_NONE = 0,
+  NSX_DRIVE_POISSON = 1,  /* k ~ Poisson(inputs*rate*dt) events of drive_weight, exc channel */
+  NSX_DRIVE_GAUSSIAN = 2, /* I += drive_mean + drive_sigma * N(0,1), fresh every step       */
+  NSX_DRIVE_OU = 3        /* Ornstein-Uhlenbeck current (mean, sigma, tau), exact update     */
+} nsx_drive_kind;
+
+typedef enum nsx_plasticity_kind {
+  NSX_PLAST_NONE = 0,
+  NSX_PLAST_PAIR = 1,   /* StdpRule + StdpTrace as engine.rs:496-537 applies them, E->E only */
+  NSX_PLAST_TRIPLET = 2 /* online triplet rule (Pfister & Gerstner 2006), E->E only          */
+} nsx_plasticity_kind;
+
+/* CROSSING: standard HH, the membrane equation of hodgkin_huxley.rs:6-9, spike = upward
+ * crossing of hh_spike_v.  REFERENCE_RESET: HodgkinHuxleyNeuron::step as written
+ * (hodgkin_huxley.rs:46-96): dv = (I - i_na - i_k - i_l)/C_m dt with i_x = g_x (E_x - V), and
+ * V >= -55 -> V := -70. */
+typedef enum nsx_hh_spike_mode {
+  NSX_HH_SPIKE_CROSSING = 0,
+  NSX_HH_SPIKE_REFERENCE_RESET = 1
+} nsx_hh_spike_mode;
+
+/* Everything SimulationEngine + NeuronModelParams + SynapseDynamics + PlasticityConfig fix for
+ * one network.  Fill with nsx_config_init() and override fields. */
+typedef struct nsx_config {
+  uint32_t struct_size; /* = sizeof(nsx_config); checked                               */
+  uint32_t model;       /* nsx_model                                                     */
+  uint64_t n_neurons;
+  uint64_t n_exc;
+  double dt_ms;         /* SimulationEngine.dt (engine.rs:45)                            */
+  uint32_t v_substeps;  /* Izhikevich V sub-steps per dt (1 = reference)                 */
+  uint32_t ring_slots;  /* D; every delay d (in steps) must satisfy 1 <= d <= D-1         */
+  uint32_t frac_bits;   /* S: arrivals are summed as int64 rint(w * 2^S)                 */
+  uint32_t record_raster; /* 1: keep every (step, neuron) spike on the device            */
+  uint64_t seed;        /* Philox4x32-10 key for drives, jitter and generators           */
+  uint64_t raster_capacity; /* spikes kept when record_raster = 1                        */
+  /* LIF (NeuronModelParams::Lif) */
+  float lif_v_rest, lif_v_reset, lif_v_th, lif_tau_m, lif_t_ref_ms, lif_r;
+  /* synaptic time constants: LIF current / HH conductance decay (SynapseDynamics) */
+  float tau_exc_ms, tau_inh_ms, tau_nmda_ms;
+  /* Izhikevich per type: [0] excitatory, [1] inhibitory (NeuronModelParams::Izhikevich) */
+  float izh_a[2], izh_b[2], izh_c[2], izh_d[2];
+  float izh_c_jitter;      /* c = c0 + izh_c_jitter * r^2, r ~ U[0,1) per neuron          */
+  float izh_d_jitter_frac; /* d = d0 * (1 - izh_d_jitter_frac * r^2)                       */
+  /* Hodgkin-Huxley (NeuronModelParams::HodgkinHuxley) and conductance synapses */
+  float hh_g_na, hh_g_k, hh_g_l, hh_e_na, hh_e_k, hh_e_l, hh_c_m;
+  float hh_spike_v;        /* crossing threshold for NSX_HH_SPIKE_CROSSING                */
+  uint32_t hh_spike_mode;  /* nsx_hh_spike_mode                                           */
+  float e_exc, e_inh;      /* reversal potentials (AMPA/NMDA 0 mV, GABA-A -70 mV)         */
+  float gmax_ampa, gmax_nmda, gmax_gaba; /* SynapseDynamics::conductance_max             */
+  float mg_mM;             /* [Mg2+] for the NMDA block (types.rs:88-94); 0 = no NMDA      */
+  uint32_t hh_substeps;    /* plain HH Euler sub-steps per dt                              */
+  /* drive */
+  uint32_t drive;          /* nsx_drive_kind                                             */
+  uint32_t drive_inputs;   /* Poisson: independent inputs per neuron                       */
+  float drive_rate_hz;     /* Poisson: rate of each input                                  */
+  float drive_weight;      /* Poisson: weight of one input event (exc channel units)       */
+  float drive_mean, drive_sigma, drive_tau_ms; /* Gaussian / OU                           */
+  /* plasticity (PlasticityConfig.stdp = StdpRule, plasticity.rs:8-28) */
+  uint32_t plasticity;     /* nsx_plasticity_kind                                        */
+  float a_plus, a_minus, tau_plus_ms, tau_minus_ms, w_min, w_max;
+  float tau_y_ms, a3_plus; /* triplet                                                      */
+  uint32_t _reserved[8];
+} nsx_config;
+
+/* Per-call statistics (SimulationStats, engine.rs:21-33, device-side subset). */
+typedef struct nsx_step_stats {
+  uint64_t steps_done;    /* steps advanced by this call                                  */
+  uint64_t step_index;    /* absolute index of the next step                              */
+  uint64_t spikes;        /* spikes emitted by this call                                  */
+  uint64_t events;        /* synaptic events = sum of out-degree of spiking rows          */
+  uint64_t total_spikes;  /* since creation                                               */
+  uint64_t total_events;
+  double sim_time_ms;     /* (step_index) * dt                                            */
+  double wall_ms;         /* device time of this call (hipEvents)                         */
+} nsx_step_stats;
+
+/* SoA neuron state view (NeuronArray, src/neuron/mod.rs:127-143, FP32 on the device).
+ * Any pointer may be NULL (that field is skipped).  Arrays hold n_neurons entries except
+ * the ring (ring_slots * channels * n_neurons int64, slot-major) and the trace arrays. */
+typedef struct nsx_state_view {
+  float* v;          /* membrane_potential                                               */
+  float* u;          /* Izhikevich recovery_variable                                      */
+  float* m;          /* hh_m                                                              */
+  float* h;          /* hh_h                                                              */
+  float* n;          /* hh_n                                                              */
+  float* syn0;       /* LIF: I_exc; HH: g_ampa                                            */
+  float* syn1;       /* LIF: I_inh; HH: g_gaba                                            */
+  float* syn2;       /* HH: g_nmda                                                        */
+  float* drive;      /* OU current                                                        */
+  int32_t* refractory; /* refractory_counter                                              */
+  int32_t* last_spike_step; /* step of last spike (-2^30 = never)                         */
+  uint32_t* spike_count;    /* spike_count                                                */
+  float* pre_trace;  /* lazy pair-STDP pre trace value at pre_step                       */
+  int32_t* pre_step;
+  float* post_trace;
+  int32_t* post_step;
+  float* post_trace2; /* triplet slow post trace                                         */
+  int64_t* ring;     /* delay ring, ring_slots*channels*n_neurons                         */
+} nsx_state_view;
+
+/* ---- library ---------------------------------------------------------------------------- */
+uint32_t nsx_abi_version(void);
+const char* nsx_status_string(nsx_status s);
+/* Last error text of ctx, or of the calling thread when ctx is NULL. */
+const char* nsx_last_error(const void* ctx);
+/* Number of gfx950 devices visible (GpuBackend::new's adapter request, gpu_backend.rs:102-111). */
+nsx_status nsx_device_count(int* count);
+
+typedef struct nsx_ctx nsx_ctx;
+
+/* ---- device-resident engine (SimulationEngine::new/step/simulate_ms, engine.rs:83-797) ----- */
+void nsx_config_init(nsx_config* cfg, uint32_t model);
+nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out);
+void nsx_destroy(nsx_ctx* ctx);
+
+/* Network::finalize (src/network/mod.rs:79-106) on the device: synapses given in insertion
+ * order as parallel arrays; builds the CSR by a stable counting sort on src, permuting col,
+ * weight and delay with it.  delay is in steps. */
+nsx_status nsx_build_csr(nsx_ctx* ctx, const uint32_t* src, const uint32_t* dst,
+                         const float* weight, const uint8_t* delay, uint64_t m);
+/* Upload an already-built CSR (GpuBackend::upload_csr, gpu_backend.rs:672-687, with 64-bit
+ * row_ptr and per-synapse delays).  Validated on the device. */
+nsx_status nsx_upload_csr(nsx_ctx* ctx, const uint64_t* row_ptr, const uint32_t* col,
+                          const float* weight, const uint8_t* delay, uint64_t n, uint64_t m);
+/* Fixed in-degree generator: every target j draws k_in presynaptic sources uniformly from
+ * [0,n) \ {j}; weights U[w_exc_lo,w_exc_hi) from excitatory and U[w_inh_lo,w_inh_hi) from
+ * inhibitory sources; delays uniform on {d_lo..d_hi} steps.  Insertion order is (j, k)
+ * target-major; the CSR is then built exactly as nsx_build_csr would. */
+nsx_status nsx_generate_fixed_indegree(nsx_ctx* ctx, uint32_t k_in, float w_exc_lo,
+                                       float w_exc_hi, float w_inh_lo, float w_inh_hi,
+                                       uint32_t d_lo, uint32_t d_hi);
+nsx_status nsx_read_csr(nsx_ctx* ctx, uint64_t* row_ptr, uint32_t* col, float* weight,
+                        uint8_t* delay);
+nsx_status nsx_network_size(nsx_ctx* ctx, uint64_t* n, uint64_t* m);
+
+/* Initial state (NeuronState::new, src/neuron/mod.rs:45-60): v = resting potential (LIF v_rest,
+ * Izhikevich c, HH -65 as HodgkinHuxleyNeuron::reset_state), u = 0, gates (0.05, 0.6, 0.32),
+ * no spike yet (last_spike_step = -2^30).  Called by nsx_create. */
+nsx_status nsx_reset_state(nsx_ctx* ctx);
+nsx_status nsx_upload_state(nsx_ctx* ctx, const nsx_state_view* s);
+nsx_status nsx_read_state(nsx_ctx* ctx, nsx_state_view* s);
+nsx_status nsx_read_weights(nsx_ctx* ctx, float* weight, uint64_t m);
+/* Step index of the next step (net.time / dt).  Drives are keyed by (seed, neuron, step), so a
+ * checkpoint restores it together with the state for an exact resume. */
+nsx_status nsx_get_step(nsx_ctx* ctx, uint64_t* step);
+nsx_status nsx_set_step(nsx_ctx* ctx, uint64_t step);
+/* Add a one-off current to neuron i for the next step (SimulationEngine::stimulate). */
+nsx_status nsx_stimulate(nsx_ctx* ctx, uint64_t neuron, float current);
+
+/* Advance n_steps steps (engine.rs:254-680, device-resident).  stats may be NULL. */
+nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* stats);
+nsx_status nsx_simulate_ms(nsx_ctx* ctx, double duration_ms, nsx_step_stats* stats);
+/* Use a captured hipGraph of `steps_per_graph` steps for nsx_step (0 = direct launches). */
+nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph);
+
+/* Raster (record_raster=1): (step, neuron) pairs in step order; within a step, neuron order.
+ * *n_out = spikes available; at most cap are copied.  clear=1 empties the raster. */
+nsx_status nsx_read_raster(nsx_ctx* ctx, uint32_t* steps, uint32_t* neurons, uint64_t cap,
+                           uint64_t* n_out, int clear);
+/* Per-step spike counts of the last nsx_step call (cap entries at most). */
+nsx_status nsx_read_step_counts(nsx_ctx* ctx, uint32_t* counts, uint64_t cap, uint64_t* n_out);
+
+/* Kernel timing: when enabled, every launch of the named kernel class in nsx_step is bracketed
+ * by hipEvents on the stream it runs on.  kind: 0 neuron update, 1 propagation, 2 STDP-LTD. */
+nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable);
+nsx_status nsx_kernel_time(nsx_ctx* ctx, int kind, double* total_ms, uint64_t* launches,
+                           uint64_t* bytes);
+
+/* ---- GpuBackend drop-in (gpu_backend.rs), FP32, synchronous ------------------------------ */
+/* GpuBackend::new(num_neurons, num_edges) (gpu_backend.rs:98) */
+nsx_status nsx_backend_new(uint64_t num_neurons, uint64_t num_edges, nsx_ctx** out);
+/* GpuBackend::upload_initial_state (640-670); params is [resting, threshold, reset, tau_m,
+ * refractory_period, input_resistance] per neuron for LIF, [a, b, c, d, 0, 0] for Izhikevich,
+ * [g_na, g_k, g_l, e_na, e_k, e_l] for HH (C_m = 1).  Gates start at 0.05/0.6/0.32
+ * (NeuronArray::new, src/neuron/mod.rs:158-160); nsx_backend_upload_hh_gates overrides them. */
+nsx_status nsx_backend_upload_initial_state(nsx_ctx* ctx, const float* membrane_potential,
+                                            const float* recovery_variable,
+                                            const int32_t* refractory_counter,
+                                            const float* last_spike_time,
+                                            const uint32_t* spike_count, const float* params6,
+                                            const uint32_t* is_output);
+nsx_status nsx_backend_upload_hh_gates(nsx_ctx* ctx, const float* m, const float* h,
+                                       const float* n);
+/* GpuBackend::upload_csr (672-687): u32 row_ptr (n+1), u32 col, f32 weight (CSR slot order). */
+nsx_status nsx_backend_upload_csr(nsx_ctx* ctx, const uint32_t* row_ptr, const uint32_t* col,
+                                  const float* weight);
+/* GpuBackend::upload_input_current (689-691) */
+nsx_status nsx_backend_upload_input_current(nsx_ctx* ctx, const float* input_current);
+/* GpuBackend::write_uniforms (693-696) */
+nsx_status nsx_backend_write_uniforms(nsx_ctx* ctx, float dt, float sim_time);
+/* GpuBackend::set_model (705-707): 0 Lif, 1 Izhikevich, 2 HodgkinHuxley */
+nsx_status nsx_backend_set_model(nsx_ctx* ctx, uint32_t model);
+/* GpuBackend::step_neurons / step_lif (711-717, 852-912): run the neuron update of the active
+ * model on the uploaded input current and read back spiked, V, u, refractory, spike_count
+ * (any output pointer may be NULL). */
+nsx_status nsx_backend_step_neurons(nsx_ctx* ctx, uint32_t* spiked, float* membrane_potential,
+                                    float* recovery_variable, int32_t* refractory_counter,
+                                    uint32_t* spike_count);
+/* GpuBackend::step_spmv (921-981): input_current += sum over spiking sources of the weights of
+ * their CSR row (int64 fixed point, order-independent), read back input_current. */
+nsx_status nsx_backend_step_spmv(nsx_ctx* ctx, float* input_current);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSX_H */

@@ -1,0 +1,382 @@
+// nsx_csr.hip -- network construction on the device.
+//
+// Network::finalize (src/network/mod.rs:79-106) is a stable counting sort of the synapse list
+// by source: rows in source order, insertion order kept inside a row.  On the device this is a
+// stable LSD radix sort of (source, insertion index) pairs; col / weight / delay are then
+// gathered through the permutation, so all three end up in CSR slot order (the reference
+// permutes only adjacency_indices, see engine.rs:244-247).  Row pointers are 64-bit.
+#include <hipcub/hipcub.hpp>
+
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "nsx_device.h"
+#include "nsx_internal.h"
+
+namespace nsx {
+
+__global__ void k_check_list(const uint32_t* src, const uint32_t* dst, const uint8_t* delay,
+                             uint64_t m, uint32_t n, uint32_t D, unsigned int* bad) {
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  if (src[k] >= n || dst[k] >= n || delay[k] < 1u || delay[k] >= D) atomicOr(bad, 1u);
+}
+
+__global__ void k_iota(uint32_t* v, uint64_t m) {
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) v[k] = (uint32_t)k;
+}
+
+// row_ptr[r] = first sorted position whose key >= r (keys sorted ascending)
+__global__ void k_row_ptr(const uint32_t* keys, uint64_t m, uint32_t n, uint64_t* row_ptr) {
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > m) return;
+  uint32_t lo = (k == 0) ? 0u : keys[k - 1] + 1u;
+  uint32_t hi = (k == m) ? n : keys[k];
+  for (uint32_t r = lo; r <= hi && r <= n; ++r) row_ptr[r] = k;
+  if (k == m && m == 0)
+    for (uint32_t r = 0; r <= n; ++r) row_ptr[r] = 0;
+}
+
+__global__ void k_gather(const uint32_t* perm, const uint32_t* dst_in, const float* w_in,
+                         const uint8_t* d_in, uint64_t m, uint32_t* col, float* w,
+                         uint8_t* delay) {
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  uint32_t q = perm[k];
+  col[k] = dst_in[q];
+  w[k] = w_in[q];
+  delay[k] = d_in[q];
+}
+
+// Fixed in-degree generator: synapse (j, k) from Philox(seed; j, k, CONN).
+__global__ void k_gen_fixed_indegree(uint64_t seed, uint32_t n, uint32_t n_exc, uint32_t k_in,
+                                     float w_exc_lo, float w_exc_hi, float w_inh_lo,
+                                     float w_inh_hi, uint32_t d_lo, uint32_t d_span,
+                                     uint32_t* src, uint32_t* dst, float* w, uint8_t* delay) {
+  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t m = (uint64_t)n * k_in;
+  if (idx >= m) return;
+  uint32_t j = (uint32_t)(idx / k_in), k = (uint32_t)(idx % k_in);
+  U4 r = draw(seed, j, k, kStreamConn);
+  uint32_t s = (uint32_t)(((uint64_t)r.x * (uint64_t)(n - 1)) >> 32);
+  if (s >= j) s += 1;
+  float u = u01(r.y);
+  const bool exc = s < n_exc;
+  float lo = exc ? w_exc_lo : w_inh_lo, hi = exc ? w_exc_hi : w_inh_hi;
+  float span = hi - lo;
+  float x = span * u;
+  src[idx] = s;
+  dst[idx] = j;
+  w[idx] = lo + x;
+  delay[idx] = (uint8_t)(d_lo + (uint32_t)(((uint64_t)r.z * d_span) >> 32));
+}
+
+__global__ void k_check_csr(const uint64_t* row_ptr, const uint32_t* col, const uint8_t* delay,
+                            uint32_t n, uint64_t m, uint32_t D, unsigned int* bad) {
+  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    if (row_ptr[k] > row_ptr[k + 1]) atomicOr(bad, 1u);
+  }
+  if (k == 0 && (row_ptr[0] != 0 || row_ptr[n] != m)) atomicOr(bad, 2u);
+  if (k < m) {
+    if (col[k] >= n) atomicOr(bad, 4u);
+    if (delay && (delay[k] < 1u || delay[k] >= D)) atomicOr(bad, 8u);
+  }
+}
+
+// E->E transpose: count per excitatory target, then scatter (slot, source).
+__global__ void k_csc_count(const uint64_t* row_ptr, const uint32_t* col, uint32_t n_exc,
+                            unsigned long long* cnt) {
+  uint32_t i = blockIdx.x;  // one block per excitatory source row
+  if (i >= n_exc) return;
+  for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
+    uint32_t j = col[k];
+    if (j < n_exc) atomicAdd(&cnt[j], 1ull);
+  }
+}
+
+__global__ void k_csc_fill(const uint64_t* row_ptr, const uint32_t* col, uint32_t n_exc,
+                           unsigned long long* cursor, uint32_t* slot, uint32_t* srcs) {
+  uint32_t i = blockIdx.x;
+  if (i >= n_exc) return;
+  for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
+    uint32_t j = col[k];
+    if (j < n_exc) {
+      unsigned long long p = atomicAdd(&cursor[j], 1ull);
+      slot[p] = (uint32_t)k;
+      srcs[p] = i;
+    }
+  }
+}
+
+static inline uint32_t blocks_for(uint64_t m, uint32_t b) { return (uint32_t)((m + b - 1) / b); }
+
+static nsx_status free_network(nsx_ctx* ctx) {
+  dev_free(ctx, ctx->row_ptr);
+  dev_free(ctx, ctx->col);
+  dev_free(ctx, ctx->w);
+  dev_free(ctx, ctx->delay);
+  dev_free(ctx, ctx->csc_ptr);
+  dev_free(ctx, ctx->csc_slot);
+  dev_free(ctx, ctx->csc_src);
+  ctx->row_ptr = nullptr;
+  ctx->col = nullptr;
+  ctx->w = nullptr;
+  ctx->delay = nullptr;
+  ctx->csc_ptr = nullptr;
+  ctx->csc_slot = nullptr;
+  ctx->csc_src = nullptr;
+  ctx->has_network = false;
+  ctx->m = 0;
+  ctx->csc_m = 0;
+  return NSX_OK;
+}
+
+static void publish_network(nsx_ctx* ctx) {
+  ctx->d.row_ptr = ctx->row_ptr;
+  ctx->d.col = ctx->col;
+  ctx->d.w = ctx->w;
+  ctx->d.delay = ctx->delay;
+  ctx->d.csc_ptr = ctx->csc_ptr;
+  ctx->d.csc_slot = ctx->csc_slot;
+  ctx->d.csc_src = ctx->csc_src;
+  ctx->has_network = true;
+  // a captured graph holds the old pointers
+  if (ctx->graph_exec) {
+    (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+  }
+  if (ctx->graph) {
+    (void)hipGraphDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+}
+
+nsx_status build_csc_device(nsx_ctx* ctx) {
+  const uint32_t ne = (uint32_t)ctx->cfg.n_exc;
+  nsx_status s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->csc_ptr, ((size_t)ne + 1) * 8, true))) return s;
+  if (ne == 0) return NSX_OK;
+  unsigned long long* cnt = nullptr;
+  if ((s = dev_alloc(ctx, (void**)&cnt, (size_t)ne * 8, true))) return s;
+  hipLaunchKernelGGL(k_csc_count, dim3(ne), dim3(128), 0, ctx->stream, ctx->row_ptr, ctx->col,
+                     ne, cnt);
+  NSX_HIP(ctx, hipGetLastError());
+  // exclusive scan into csc_ptr[1..]
+  size_t tmp_bytes = 0;
+  NSX_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, cnt,
+                                                 (unsigned long long*)ctx->csc_ptr + 1, (int)ne,
+                                                 ctx->stream));
+  void* tmp = nullptr;
+  if ((s = dev_alloc(ctx, &tmp, tmp_bytes, false))) return s;
+  NSX_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, cnt,
+                                                 (unsigned long long*)ctx->csc_ptr + 1, (int)ne,
+                                                 ctx->stream));
+  uint64_t tot = 0;
+  NSX_HIP(ctx, hipMemcpyAsync(&tot, ctx->csc_ptr + ne, 8, hipMemcpyDeviceToHost, ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, tmp);
+  if ((s = dev_alloc(ctx, (void**)&ctx->csc_slot, tot * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->csc_src, tot * 4, false))) return s;
+  NSX_HIP(ctx, hipMemcpyAsync(cnt, ctx->csc_ptr, (size_t)ne * 8, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+  hipLaunchKernelGGL(k_csc_fill, dim3(ne), dim3(128), 0, ctx->stream, ctx->row_ptr, ctx->col, ne,
+                     cnt, ctx->csc_slot, ctx->csc_src);
+  NSX_HIP(ctx, hipGetLastError());
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, cnt);
+  ctx->csc_m = tot;
+  return NSX_OK;
+}
+
+// Sort (src, insertion index) stably, build row_ptr, gather col/w/delay. Consumes the inputs.
+nsx_status build_csr_device(nsx_ctx* ctx, uint32_t* d_src, uint32_t* d_dst, float* d_w,
+                            uint8_t* d_delay, uint64_t m) {
+  const uint32_t n = (uint32_t)ctx->n;
+  nsx_status s;
+  if (m >= (1ull << 31))
+    return set_error(ctx, NSX_E_INVALID, "more than 2^31-1 synapses per build call");
+  free_network(ctx);
+  uint32_t *keys_out = nullptr, *perm_in = nullptr, *perm_out = nullptr;
+  if ((s = dev_alloc(ctx, (void**)&keys_out, m * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&perm_in, m * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&perm_out, m * 4, false))) return s;
+  if (m) {
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(m, 256)), dim3(256), 0, ctx->stream, perm_in, m);
+    NSX_HIP(ctx, hipGetLastError());
+  }
+  int end_bit = 1;
+  while (end_bit < 32 && (1ull << end_bit) < (uint64_t)n) ++end_bit;
+  size_t tmp_bytes = 0;
+  NSX_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_src, keys_out, perm_in,
+                                                  perm_out, (int)m, 0, end_bit, ctx->stream));
+  void* tmp = nullptr;
+  if ((s = dev_alloc(ctx, &tmp, tmp_bytes, false))) return s;
+  if (m)
+    NSX_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, d_src, keys_out, perm_in,
+                                                    perm_out, (int)m, 0, end_bit, ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, tmp);
+  dev_free(ctx, perm_in);
+  if ((s = dev_alloc(ctx, (void**)&ctx->row_ptr, ((size_t)n + 1) * 8, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->col, m * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->w, m * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->delay, m, false))) return s;
+  hipLaunchKernelGGL(k_row_ptr, dim3(blocks_for(m + 1, 256)), dim3(256), 0, ctx->stream,
+                     keys_out, m, n, ctx->row_ptr);
+  NSX_HIP(ctx, hipGetLastError());
+  if (m) {
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(m, 256)), dim3(256), 0, ctx->stream, perm_out,
+                       d_dst, d_w, d_delay, m, ctx->col, ctx->w, ctx->delay);
+    NSX_HIP(ctx, hipGetLastError());
+  }
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, keys_out);
+  dev_free(ctx, perm_out);
+  ctx->m = m;
+  if (ctx->cfg.plasticity == NSX_PLAST_PAIR && (s = build_csc_device(ctx))) return s;
+  publish_network(ctx);
+  return NSX_OK;
+}
+
+nsx_status validate_csr_device(nsx_ctx* ctx) {
+  unsigned int* bad = nullptr;
+  nsx_status s;
+  if ((s = dev_alloc(ctx, (void**)&bad, 4, true))) return s;
+  uint64_t cover = std::max<uint64_t>(ctx->m, ctx->n);
+  hipLaunchKernelGGL(k_check_csr, dim3(blocks_for(cover + 1, 256)), dim3(256), 0, ctx->stream,
+                     ctx->row_ptr, ctx->col, ctx->delay, (uint32_t)ctx->n, ctx->m,
+                     (uint32_t)ctx->D, bad);
+  NSX_HIP(ctx, hipGetLastError());
+  unsigned int hb = 0;
+  NSX_HIP(ctx, hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, bad);
+  if (hb & 1u) return set_error(ctx, NSX_E_BADCSR, "row_ptr is not monotone");
+  if (hb & 2u) return set_error(ctx, NSX_E_BADCSR, "row_ptr[0] != 0 or row_ptr[n] != m");
+  if (hb & 4u) return set_error(ctx, NSX_E_BADCSR, "column index out of range");
+  if (hb & 8u) return set_error(ctx, NSX_E_BADCSR, "delay outside [1, ring_slots-1]");
+  return NSX_OK;
+}
+
+}  // namespace nsx
+
+using namespace nsx;
+
+extern "C" {
+
+nsx_status nsx_build_csr(nsx_ctx* ctx, const uint32_t* src, const uint32_t* dst,
+                         const float* weight, const uint8_t* delay, uint64_t m) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (m && (!src || !dst || !weight || !delay))
+    return set_error(ctx, NSX_E_INVALID, "NULL synapse array");
+  uint32_t *ds = nullptr, *dd = nullptr;
+  float* dw = nullptr;
+  uint8_t* dl = nullptr;
+  unsigned int* bad = nullptr;
+  nsx_status s;
+  if ((s = dev_alloc(ctx, (void**)&ds, m * 4, false)) ||
+      (s = dev_alloc(ctx, (void**)&dd, m * 4, false)) ||
+      (s = dev_alloc(ctx, (void**)&dw, m * 4, false)) ||
+      (s = dev_alloc(ctx, (void**)&dl, m, false)) || (s = dev_alloc(ctx, (void**)&bad, 4, true)))
+    return s;
+  if (m) {
+    NSX_HIP(ctx, hipMemcpyAsync(ds, src, m * 4, hipMemcpyHostToDevice, ctx->stream));
+    NSX_HIP(ctx, hipMemcpyAsync(dd, dst, m * 4, hipMemcpyHostToDevice, ctx->stream));
+    NSX_HIP(ctx, hipMemcpyAsync(dw, weight, m * 4, hipMemcpyHostToDevice, ctx->stream));
+    NSX_HIP(ctx, hipMemcpyAsync(dl, delay, m, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_check_list, dim3(blocks_for(m, 256)), dim3(256), 0, ctx->stream, ds, dd,
+                       dl, m, (uint32_t)ctx->n, (uint32_t)ctx->D, bad);
+    NSX_HIP(ctx, hipGetLastError());
+  }
+  unsigned int hb = 0;
+  NSX_HIP(ctx, hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, bad);
+  if (hb) {
+    dev_free(ctx, ds); dev_free(ctx, dd); dev_free(ctx, dw); dev_free(ctx, dl);
+    return set_error(ctx, NSX_E_BADCSR,
+                     "synapse with source/target >= n_neurons or delay outside [1, ring_slots-1]");
+  }
+  s = build_csr_device(ctx, ds, dd, dw, dl, m);
+  dev_free(ctx, ds); dev_free(ctx, dd); dev_free(ctx, dw); dev_free(ctx, dl);
+  return s;
+}
+
+nsx_status nsx_generate_fixed_indegree(nsx_ctx* ctx, uint32_t k_in, float w_exc_lo,
+                                       float w_exc_hi, float w_inh_lo, float w_inh_hi,
+                                       uint32_t d_lo, uint32_t d_hi) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (ctx->n < 2) return set_error(ctx, NSX_E_INVALID, "generator needs n_neurons >= 2");
+  if (d_lo < 1 || d_hi < d_lo || d_hi >= (uint32_t)ctx->D)
+    return set_error(ctx, NSX_E_INVALID, "delays must satisfy 1 <= d_lo <= d_hi <= ring_slots-1");
+  const uint64_t m = ctx->n * (uint64_t)k_in;
+  uint32_t *ds = nullptr, *dd = nullptr;
+  float* dw = nullptr;
+  uint8_t* dl = nullptr;
+  nsx_status s;
+  if ((s = dev_alloc(ctx, (void**)&ds, m * 4, false)) ||
+      (s = dev_alloc(ctx, (void**)&dd, m * 4, false)) ||
+      (s = dev_alloc(ctx, (void**)&dw, m * 4, false)) || (s = dev_alloc(ctx, (void**)&dl, m, false)))
+    return s;
+  if (m) {
+    hipLaunchKernelGGL(k_gen_fixed_indegree, dim3(blocks_for(m, 256)), dim3(256), 0, ctx->stream,
+                       ctx->cfg.seed, (uint32_t)ctx->n, (uint32_t)ctx->cfg.n_exc, k_in, w_exc_lo,
+                       w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_hi - d_lo + 1, ds, dd, dw, dl);
+    NSX_HIP(ctx, hipGetLastError());
+  }
+  s = build_csr_device(ctx, ds, dd, dw, dl, m);
+  dev_free(ctx, ds); dev_free(ctx, dd); dev_free(ctx, dw); dev_free(ctx, dl);
+  return s;
+}
+
+nsx_status nsx_upload_csr(nsx_ctx* ctx, const uint64_t* row_ptr, const uint32_t* col,
+                          const float* weight, const uint8_t* delay, uint64_t n, uint64_t m) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (n != ctx->n) return set_error(ctx, NSX_E_INVALID, "n does not match the context");
+  if (!row_ptr || (m && (!col || !weight || !delay)))
+    return set_error(ctx, NSX_E_INVALID, "NULL CSR array");
+  free_network(ctx);
+  nsx_status s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->row_ptr, (n + 1) * 8, false)) ||
+      (s = dev_alloc(ctx, (void**)&ctx->col, m * 4, false)) ||
+      (s = dev_alloc(ctx, (void**)&ctx->w, m * 4, false)) ||
+      (s = dev_alloc(ctx, (void**)&ctx->delay, m, false)))
+    return s;
+  NSX_HIP(ctx, hipMemcpyAsync(ctx->row_ptr, row_ptr, (n + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+  if (m) {
+    NSX_HIP(ctx, hipMemcpyAsync(ctx->col, col, m * 4, hipMemcpyHostToDevice, ctx->stream));
+    NSX_HIP(ctx, hipMemcpyAsync(ctx->w, weight, m * 4, hipMemcpyHostToDevice, ctx->stream));
+    NSX_HIP(ctx, hipMemcpyAsync(ctx->delay, delay, m, hipMemcpyHostToDevice, ctx->stream));
+  }
+  ctx->m = m;
+  if ((s = validate_csr_device(ctx))) {
+    free_network(ctx);
+    return s;
+  }
+  if (ctx->cfg.plasticity == NSX_PLAST_PAIR && (s = build_csc_device(ctx))) return s;
+  publish_network(ctx);
+  return NSX_OK;
+}
+
+nsx_status nsx_read_csr(nsx_ctx* ctx, uint64_t* row_ptr, uint32_t* col, float* weight,
+                        uint8_t* delay) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (!ctx->has_network) return set_error(ctx, NSX_E_STATE, "no network loaded");
+  if (row_ptr)
+    NSX_HIP(ctx, hipMemcpyAsync(row_ptr, ctx->row_ptr, (ctx->n + 1) * 8, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  if (ctx->m) {
+    if (col) NSX_HIP(ctx, hipMemcpyAsync(col, ctx->col, ctx->m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (weight) NSX_HIP(ctx, hipMemcpyAsync(weight, ctx->w, ctx->m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (delay) NSX_HIP(ctx, hipMemcpyAsync(delay, ctx->delay, ctx->m, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return NSX_OK;
+}
+
+}  // extern "C"

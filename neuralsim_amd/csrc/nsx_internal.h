@@ -1,0 +1,153 @@
+// nsx_internal.h -- host-side context of the simulation core and kernel parameter blocks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/nsx.h"
+
+namespace nsx {
+
+constexpr int kMaxPoisson = 32;
+constexpr uint32_t kBlock = 256;
+
+// Everything a step kernel reads that is constant over a run (passed by value).
+struct StepParams {
+  uint32_t n, n_exc, D, ch;
+  uint32_t nblk;  // neuron blocks of kBlock neurons (spike segments)
+  uint64_t seed;
+  double scale_in, scale_out;  // 2^S, 2^-S
+  // LIF
+  float v_rest, v_reset, v_th, lif_r, lif_k;
+  int32_t refr_steps;
+  // synaptic decay factors exp(-dt/tau)
+  float dec_e, dec_i, dec_n;
+  // Izhikevich
+  float izh_a[2], izh_b[2];
+  float izh_h, dtf;
+  uint32_t substeps;
+  // HH
+  float g_na, g_k, g_l, e_na, e_k, e_l, hh_kv, hh_spike_v;
+  float e_exc, e_inh, gmax_ampa, gmax_nmda, gmax_gaba, mgc;
+  uint32_t hh_reset_mode, use_nmda;
+  // drive
+  float drive_mean, drive_sigma, drive_weight, ou_k1, ou_k2, ou_k3;
+  uint32_t pthr[kMaxPoisson];
+  int32_t npthr;
+  // pair STDP
+  float a_plus, a_minus, w_min, w_max;
+  int32_t ptab_n, mtab_n;
+  uint32_t step_counts_cap;
+  uint32_t raster_on;
+  uint64_t raster_cap;
+};
+
+// Device pointers used by the step kernels.
+struct DevPtrs {
+  float *v, *u, *gm, *gh, *gn, *s0, *s1, *s2, *drv, *c, *d;
+  int32_t *refr, *last;
+  uint32_t* count;
+  float *pre_v, *post_v;
+  int32_t *pre_s, *post_s;
+  long long* ring;
+  const float *ptab, *mtab;
+  uint32_t* spikes;  // per-block spike segments: block b's spikes at [b*kBlock, +bcnt[b])
+  uint32_t* bcnt;    // spikes of each neuron block in the current step
+  unsigned long long* part;  // per-block running totals, kPart* planes of nblk entries
+  uint32_t* nspk;    // GpuBackend drop-in spike list length
+  uint32_t *raster_step, *raster_id;
+  unsigned long long* counters;  // see kCnt*
+  uint32_t* step_counts;
+  int32_t* tbase;
+  // network
+  const uint64_t* row_ptr;
+  const uint32_t* col;
+  float* w;
+  const uint8_t* delay;
+  const uint64_t* csc_ptr;
+  const uint32_t* csc_slot;
+  const uint32_t* csc_src;
+};
+
+// planes of DevPtrs::part (each nblk entries; entry b written only by the wave owning block b)
+enum : int { kPartSpikes = 0, kPartEvents = 1, kPartLtp = 2, kPartLtd = 3, kNumPart = 4 };
+
+// counters[]
+enum : int {
+  kCntSpikes = 0,
+  kCntEvents = 1,
+  kCntLtdSlots = 2,
+  kCntLtpSlots = 3,
+  kCntRaster = 4,
+  kCntOverflow = 5,
+  kCntExtraInput = 6,
+  kNumCounters = 8
+};
+
+}  // namespace nsx
+
+struct nsx_ctx {
+  nsx_config cfg;
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t n = 0, m = 0;
+  int ch = 1, D = 2;
+  bool has_network = false;
+  uint64_t t = 0;          // next step index
+  int32_t tbase_host = 0;  // value held in d.tbase
+  nsx::StepParams p{};
+  nsx::DevPtrs d{};
+  // owned device allocations
+  std::vector<void*> allocs;
+  uint64_t* row_ptr = nullptr;
+  uint32_t* col = nullptr;
+  float* w = nullptr;
+  uint8_t* delay = nullptr;
+  uint64_t* csc_ptr = nullptr;
+  uint32_t* csc_slot = nullptr;
+  uint32_t* csc_src = nullptr;
+  uint64_t csc_m = 0;
+  // graph
+  uint32_t graph_steps = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  // profiling
+  bool profiling = false;
+  double kt_ms[3] = {0, 0, 0};
+  uint64_t kt_launches[3] = {0, 0, 0};
+  std::vector<hipEvent_t> ev_pool;
+  uint32_t last_call_steps = 0;
+  std::vector<unsigned long long> part_host;
+  uint64_t part_delta[4] = {0, 0, 0, 0};
+  uint64_t last_call_first_step = 0;
+  // GpuBackend drop-in state
+  bool backend_mode = false;
+  float* b_input = nullptr;     // uploaded input_current (f32)
+  float* b_params = nullptr;    // 6 floats per neuron
+  uint32_t* b_spiked = nullptr;
+  uint32_t* b_row_ptr32 = nullptr;
+  float b_dt = 0.5f, b_time = 0.0f;
+  uint32_t b_model = 0;
+};
+
+// helpers shared by translation units
+namespace nsx {
+nsx_status set_error(nsx_ctx* ctx, nsx_status s, const std::string& msg);
+nsx_status hip_error(nsx_ctx* ctx, hipError_t e, const char* what);
+nsx_status dev_alloc(nsx_ctx* ctx, void** p, size_t bytes, bool zero);
+void dev_free(nsx_ctx* ctx, void* p);
+nsx_status build_csr_device(nsx_ctx* ctx, uint32_t* d_src, uint32_t* d_dst, float* d_w,
+                            uint8_t* d_delay, uint64_t m);
+nsx_status build_csc_device(nsx_ctx* ctx);
+nsx_status validate_csr_device(nsx_ctx* ctx);
+}  // namespace nsx
+
+#define NSX_HIP(ctx, expr)                                          \
+  do {                                                              \
+    hipError_t _e = (expr);                                         \
+    if (_e != hipSuccess) return nsx::hip_error((ctx), _e, #expr);  \
+  } while (0)

@@ -1,0 +1,481 @@
+// nsx_step.hip -- per-step kernels of the simulation core (gfx950, wave64).
+//
+// One step t is three launches on one stream:
+//   k_update     one thread per neuron: read+clear ring slot t, external drive, model update
+//                (LIF / Izhikevich / HH), spike compaction (wave ballot -> one atomic per wave),
+//                pre-synaptic trace bump.             Replaces SimulationEngine::step phase 1
+//                (engine.rs:267-391) and the lif/izhikevich/hh_update.wgsl shaders.
+//   k_propagate  one wavefront per spiking row: coalesced col/weight/delay streams, int64
+//                fixed-point atomic add into ring[(t+d) mod D][channel][target]; pair-STDP
+//                potentiation of E->E synapses of the row in the same pass.  Replaces the
+//                propagation loop engine.rs:414-473 (delays honoured) and spmv.wgsl.
+//   k_ltd        one wavefront per spiking excitatory neuron: depression of its E->E in-synapses
+//                through the transpose index, then its post-synaptic trace bump.  Together with
+//                k_propagate this is engine.rs:496-537 with per-neuron traces.
+// Kernels read the step index as *tbase + off so that a captured hipGraph of G steps can be
+// replayed (k_advance moves tbase by G at the end of the graph).
+#include "nsx_device.h"
+#include "nsx_internal.h"
+
+namespace nsx {
+
+__device__ inline float trace_at(const float* tab, int32_t tab_n, int32_t k) {
+  return (k >= 0 && k < tab_n) ? tab[k] : 0.0f;
+}
+
+// Per-step drive value for neuron i (value added to channel 0 / the input current).
+template <int DRIVE>
+__device__ inline void drive_draw(const StepParams& p, uint32_t i, int32_t t, float& a0,
+                                  float& z) {
+  if (DRIVE == NSX_DRIVE_NONE) return;
+  U4 r = draw(p.seed, i, (uint32_t)t, kStreamDrive);
+  if (DRIVE == NSX_DRIVE_POISSON) {
+    uint32_t c = 0;
+    for (int k = 0; k < p.npthr; ++k) c += (r.x >= p.pthr[k]) ? 1u : 0u;
+    float x = (float)c * p.drive_weight;
+    a0 = a0 + x;
+  } else {
+    z = normal_(r.x, r.y);
+  }
+}
+
+// alpha = scale * x / (1 - exp(-x/10)); near x = 0 the limit k * (1 + x/20).
+__device__ inline float hh_ratio(float x, float scale, float k) {
+  if (fabsf(x) < 1e-3f) {
+    float y = x * 0.05f;
+    y = 1.0f + y;
+    return k * y;
+  }
+  float e = expf_(x * -0.1f);
+  float den = 1.0f - e;
+  float num = scale * x;
+  return num / den;
+}
+
+template <int MODEL, int DRIVE, bool PAIR>
+__global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
+  const int32_t t = *s.tbase + off;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool spiked = false;
+  if (i < p.n) {
+    const uint32_t slot = (uint32_t)t % p.D;
+    long long* r0 = s.ring + (size_t)slot * p.ch * p.n + i;
+    float a0 = dequantize(*r0, p.scale_out);
+    *r0 = 0;
+    float a1 = 0.0f;
+    if (MODEL != NSX_MODEL_IZHIKEVICH) {
+      long long* r1 = r0 + p.n;
+      a1 = dequantize(*r1, p.scale_out);
+      *r1 = 0;
+    }
+    float z = 0.0f;
+    drive_draw<DRIVE>(p, i, t, a0, z);
+    if (MODEL == NSX_MODEL_LIF) {
+      float ie = s.s0[i] * p.dec_e;
+      ie = ie + a0;
+      float ii = s.s1[i] * p.dec_i;
+      ii = ii + a1;
+      s.s0[i] = ie;
+      s.s1[i] = ii;
+      int32_t refr = s.refr[i];
+      if (refr > 0) {
+        s.refr[i] = refr - 1;
+      } else {
+        float v = s.v[i];
+        float I = ie + ii;
+        float x = v - p.v_rest;
+        x = -x;
+        float y = p.lif_r * I;
+        x = x + y;
+        x = x * p.lif_k;
+        float vn = v + x;
+        if (vn >= p.v_th) {
+          vn = p.v_reset;
+          s.refr[i] = p.refr_steps;
+          spiked = true;
+        }
+        s.v[i] = vn;
+      }
+    } else if (MODEL == NSX_MODEL_IZHIKEVICH) {
+      float I = a0;
+      if (DRIVE == NSX_DRIVE_GAUSSIAN) {
+        float q = p.drive_sigma * z;
+        q = p.drive_mean + q;
+        I = I + q;
+      }
+      float v = s.v[i], u = s.u[i];
+      if (v >= 30.0f) {
+        v = s.c[i];
+        u = u + s.d[i];
+        spiked = true;
+      } else {
+        const int ty = i < p.n_exc ? 0 : 1;
+        const float a = p.izh_a[ty], b = p.izh_b[ty];
+        const float v0 = v;
+        for (uint32_t k = 0; k < p.substeps; ++k) {
+          float t1 = 0.04f * v;
+          t1 = t1 * v;
+          float t2 = 5.0f * v;
+          float acc = t1 + t2;
+          acc = acc + 140.0f;
+          acc = acc - u;
+          acc = acc + I;
+          acc = acc * p.izh_h;
+          v = v + acc;
+        }
+        float t3 = b * v0;
+        t3 = t3 - u;
+        t3 = a * t3;
+        t3 = t3 * p.dtf;
+        u = u + t3;
+      }
+      s.v[i] = v;
+      s.u[i] = u;
+    } else {  // Hodgkin-Huxley with AMPA / GABA-A / NMDA conductances
+      const float v = s.v[i];
+      float ga = s.s0[i] * p.dec_e;
+      float xa = a0 * p.gmax_ampa;
+      ga = ga + xa;
+      float gn = s.s2[i] * p.dec_n;
+      if (p.use_nmda) {
+        float e = expf_(-0.062f * v);
+        float bl = p.mgc * e;
+        bl = 1.0f + bl;
+        float xn = a0 * p.gmax_nmda;
+        xn = xn / bl;
+        gn = gn + xn;
+      }
+      float gg = s.s1[i] * p.dec_i;
+      float xg = fabsf(a1) * p.gmax_gaba;
+      gg = gg + xg;
+      float de = p.e_exc - v, di = p.e_inh - v;
+      float isyn = ga * de;
+      float tn = gn * de;
+      isyn = isyn + tn;
+      float tg = gg * di;
+      isyn = isyn + tg;
+      float iext = 0.0f;
+      if (DRIVE == NSX_DRIVE_OU) {
+        iext = s.drv[i];
+        float o = iext * p.ou_k1;
+        o = o + p.ou_k2;
+        float q = p.ou_k3 * z;
+        s.drv[i] = o + q;
+      } else if (DRIVE == NSX_DRIVE_GAUSSIAN) {
+        float q = p.drive_sigma * z;
+        iext = p.drive_mean + q;
+      }
+      float I = iext + isyn;
+      float am = hh_ratio(v + 40.0f, 0.1f, 1.0f);
+      float bm = 4.0f * expf_((v + 65.0f) * (-1.0f / 18.0f));
+      float ah = 0.07f * expf_((v + 65.0f) * -0.05f);
+      float eh = expf_((v + 35.0f) * -0.1f);
+      eh = 1.0f + eh;
+      float bh = 1.0f / eh;
+      float an = hh_ratio(v + 55.0f, 0.01f, 0.1f);
+      float bn = 0.125f * expf_((v + 65.0f) * -0.0125f);
+      float m = s.gm[i], h = s.gh[i], nn = s.gn[i];
+      float t1, t2;
+      t1 = am * (1.0f - m); t2 = bm * m; t1 = t1 - t2; t1 = t1 * p.dtf; m = m + t1;
+      t1 = ah * (1.0f - h); t2 = bh * h; t1 = t1 - t2; t1 = t1 * p.dtf; h = h + t1;
+      t1 = an * (1.0f - nn); t2 = bn * nn; t1 = t1 - t2; t1 = t1 * p.dtf; nn = nn + t1;
+      float m3 = m * m;
+      m3 = m3 * m;
+      float ina = p.g_na * m3;
+      ina = ina * h;
+      ina = ina * (p.e_na - v);
+      float n2 = nn * nn;
+      float n4 = n2 * n2;
+      float ik = p.g_k * n4;
+      ik = ik * (p.e_k - v);
+      float il = p.g_l * (p.e_l - v);
+      // reference-reset mode keeps the reference code's current signs (hodgkin_huxley.rs:78-84);
+      // standard mode integrates the documented membrane equation (hodgkin_huxley.rs:6-9)
+      float acc;
+      if (p.hh_reset_mode) {
+        acc = I - ina;
+        acc = acc - ik;
+        acc = acc - il;
+      } else {
+        acc = I + ina;
+        acc = acc + ik;
+        acc = acc + il;
+      }
+      acc = acc * p.hh_kv;
+      float vn = v + acc;
+      if (p.hh_reset_mode) {
+        spiked = vn >= -55.0f;
+        if (spiked) vn = -70.0f;
+      } else {
+        spiked = (v < p.hh_spike_v) && (vn >= p.hh_spike_v);
+      }
+      s.v[i] = vn;
+      s.gm[i] = m;
+      s.gh[i] = h;
+      s.gn[i] = nn;
+      s.s0[i] = ga;
+      s.s1[i] = gg;
+      s.s2[i] = gn;
+    }
+    if (spiked) {
+      s.count[i] += 1u;
+      s.last[i] = t;
+      if (PAIR && i < p.n_exc) {
+        const int32_t ps = s.pre_s[i];
+        float q = trace_at(s.ptab, p.ptab_n, t - ps);
+        float x = s.pre_v[i] * q;
+        x = x * q;
+        s.pre_v[i] = x + 1.0f;
+        s.pre_s[i] = t;
+      }
+    }
+  }
+  // Block-local compaction into this block's spike segment: wave ballots, wave counts in LDS,
+  // no global atomics (one atomic per wave on a shared counter serialises at the memory-side
+  // atomic unit once tens of thousands of waves fire).
+  __shared__ uint32_t s_wcnt[kBlock / 64];
+  const unsigned long long mask = __ballot(spiked);
+  const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
+  const uint32_t nset = (uint32_t)__popcll(mask);
+  const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wcnt[wid] = nset;
+  __syncthreads();
+  uint32_t off_w = 0, tot = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kBlock / 64; ++k) {
+    const uint32_t c = s_wcnt[k];
+    off_w += k < wid ? c : 0u;
+    tot += c;
+  }
+  if (spiked) s.spikes[blockIdx.x * kBlock + off_w + below] = i;
+  if (threadIdx.x == 0) {
+    s.bcnt[blockIdx.x] = tot;
+    if (tot) s.part[kPartSpikes * p.nblk + blockIdx.x] += tot;
+  }
+  if (p.raster_on && mask) {  // test/analysis path: append (step, id) pairs
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long rb = 0;
+    if ((int)lane == leader) {
+      rb = atomicAdd(&s.counters[kCntRaster], (unsigned long long)nset);
+      atomicAdd(&s.step_counts[(uint32_t)t % p.step_counts_cap], nset);
+    }
+    rb = __shfl(rb, leader);
+    if (spiked) {
+      unsigned long long pos = rb + below;
+      if (pos < p.raster_cap) {
+        s.raster_step[pos] = (uint32_t)t;
+        s.raster_id[pos] = i;
+      }
+    }
+  }
+}
+
+// Order LDS traffic of one wavefront (all lanes write, then any lane reads).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Flattened view of up to 64 rows: lane r holds row r (start, source); incl[r] is the
+// inclusive prefix of row lengths.  Lanes then walk the concatenated slots 64 at a time.
+struct RowChunk {
+  uint32_t incl[64];
+  uint32_t src[64];
+  uint64_t beg[64];
+};
+
+// One wavefront per neuron block: all spiking rows of the block's segment, flattened so
+// that every lane streams slots (coalesced col / weight / delay), int64 fixed-point atomic
+// add into the ring slot of the arrival step, pair-STDP potentiation on E->E slots.
+template <bool PAIR>
+__global__ __launch_bounds__(kBlock) void k_propagate(StepParams p, DevPtrs s, int32_t off) {
+  __shared__ RowChunk sh[kBlock / 64];
+  const int32_t t = *s.tbase + off;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+  const uint32_t tm = (uint32_t)t % p.D;
+  const size_t plane = (size_t)p.ch * p.n;
+  RowChunk& rc = sh[wid];
+  for (uint32_t b = wave; b < p.nblk; b += nwaves) {
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
+    if (cnt == 0) continue;
+    unsigned long long ev = 0, ltp = 0;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+      const uint32_t nr = min(64u, cnt - c0);
+      uint32_t i = 0, deg = 0;
+      uint64_t beg = 0;
+      if (lane < nr) {
+        i = s.spikes[b * kBlock + c0 + lane];
+        beg = s.row_ptr[i];
+        deg = (uint32_t)(s.row_ptr[i + 1] - beg);
+      }
+      uint32_t incl = deg;
+#pragma unroll
+      for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+      }
+      const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+      rc.incl[lane] = incl;
+      rc.src[lane] = i;
+      rc.beg[lane] = beg;
+      wave_sync();
+      ev += total;
+      uint32_t r = 0;
+      for (uint32_t f = lane; f < total; f += 64) {
+        while (rc.incl[r] <= f) ++r;
+        const uint32_t excl = r ? rc.incl[r - 1] : 0u;
+        const uint64_t k = rc.beg[r] + (f - excl);
+        const uint32_t src = rc.src[r];
+        const uint32_t j = s.col[k];
+        const float w = s.w[k];
+        uint32_t sl = tm + (uint32_t)s.delay[k];
+        sl = sl >= p.D ? sl - p.D : sl;
+        const uint32_t chn = (p.ch > 1 && src >= p.n_exc) ? 1u : 0u;
+        atomicAdd((unsigned long long*)(s.ring + sl * plane + (size_t)chn * p.n + j),
+                  (unsigned long long)quantize(w, p.scale_in));
+        if (PAIR && src < p.n_exc && j < p.n_exc) {
+          const float q = trace_at(s.mtab, p.mtab_n, t - s.post_s[j]);
+          const float x = s.post_v[j] * q;
+          const float dlt = p.a_plus * x;
+          const float wn = w + dlt;
+          s.w[k] = clampf_(wn, p.w_min, p.w_max);
+          ltp += 1;
+        }
+      }
+      wave_sync();
+    }
+    if (PAIR) {
+#pragma unroll
+      for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
+    }
+    if (lane == 0) {
+      s.part[kPartEvents * p.nblk + b] += ev;
+      if (PAIR) s.part[kPartLtp * p.nblk + b] += ltp;
+    }
+  }
+}
+
+// Post-spike depression: one wavefront per neuron block, the E->E in-slots (transpose index)
+// of the block's spiking excitatory neurons flattened across lanes; then their post trace bump.
+__global__ __launch_bounds__(kBlock) void k_ltd(StepParams p, DevPtrs s, int32_t off) {
+  __shared__ RowChunk sh[kBlock / 64];
+  const int32_t t = *s.tbase + off;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+  const uint32_t nblk_e = (p.n_exc + kBlock - 1) / kBlock;  // blocks holding excitatory neurons
+  RowChunk& rc = sh[wid];
+  for (uint32_t b = wave; b < nblk_e; b += nwaves) {
+    const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
+    if (cnt == 0) continue;
+    unsigned long long slots = 0;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
+      const uint32_t nr = min(64u, cnt - c0);
+      uint32_t j = 0, deg = 0;
+      uint64_t beg = 0;
+      const bool mine = lane < nr;
+      if (mine) {
+        j = s.spikes[b * kBlock + c0 + lane];
+        if (j < p.n_exc) {
+          beg = s.csc_ptr[j];
+          deg = (uint32_t)(s.csc_ptr[j + 1] - beg);
+        }
+      }
+      uint32_t incl = deg;
+#pragma unroll
+      for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+      }
+      const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+      rc.incl[lane] = incl;
+      rc.beg[lane] = beg;
+      wave_sync();
+      slots += total;
+      uint32_t r = 0;
+      for (uint32_t f = lane; f < total; f += 64) {
+        while (rc.incl[r] <= f) ++r;
+        const uint32_t excl = r ? rc.incl[r - 1] : 0u;
+        const uint64_t e = rc.beg[r] + (f - excl);
+        const uint32_t i = s.csc_src[e];
+        const uint32_t k = s.csc_slot[e];
+        const float q = trace_at(s.ptab, p.ptab_n, t - s.pre_s[i]);
+        const float x = s.pre_v[i] * q;
+        const float dlt = p.a_minus * x;
+        const float wn = s.w[k] - dlt;
+        s.w[k] = clampf_(wn, p.w_min, p.w_max);
+      }
+      if (mine && j < p.n_exc) {
+        const int32_t ps = s.post_s[j];
+        const float q = trace_at(s.mtab, p.mtab_n, t - ps);
+        float x = s.post_v[j] * q;
+        x = x * q;
+        s.post_v[j] = x + 1.0f;
+        s.post_s[j] = t;
+      }
+      wave_sync();
+    }
+    if (lane == 0) s.part[kPartLtd * p.nblk + b] += slots;
+  }
+}
+
+__global__ void k_advance(int32_t* tbase, int32_t g) { *tbase += g; }
+
+// ---- launch helpers ------------------------------------------------------------------------
+typedef void (*UpdateFn)(StepParams, DevPtrs, int32_t);
+
+template <int MODEL, int DRIVE>
+static UpdateFn pick_pair(bool pair) {
+  return pair ? k_update<MODEL, DRIVE, true> : k_update<MODEL, DRIVE, false>;
+}
+template <int MODEL>
+static UpdateFn pick_drive(int drive, bool pair) {
+  switch (drive) {
+    case NSX_DRIVE_POISSON: return pick_pair<MODEL, NSX_DRIVE_POISSON>(pair);
+    case NSX_DRIVE_GAUSSIAN: return pick_pair<MODEL, NSX_DRIVE_GAUSSIAN>(pair);
+    case NSX_DRIVE_OU: return pick_pair<MODEL, NSX_DRIVE_OU>(pair);
+    default: return pick_pair<MODEL, NSX_DRIVE_NONE>(pair);
+  }
+}
+UpdateFn pick_update(int model, int drive, bool pair) {
+  switch (model) {
+    case NSX_MODEL_LIF: return pick_drive<NSX_MODEL_LIF>(drive, pair);
+    case NSX_MODEL_IZHIKEVICH: return pick_drive<NSX_MODEL_IZHIKEVICH>(drive, pair);
+    default: return pick_drive<NSX_MODEL_HH>(drive, pair);
+  }
+}
+
+// one wavefront per neuron block
+uint32_t prop_grid(const nsx_ctx* ctx) { return (ctx->p.nblk + kBlock / 64 - 1) / (kBlock / 64); }
+
+// Enqueue the launches of one step with offset `off` from *tbase.
+hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
+  const StepParams& p = ctx->p;
+  const bool pair = ctx->cfg.plasticity == NSX_PLAST_PAIR;
+  UpdateFn upd = pick_update((int)ctx->cfg.model, (int)ctx->cfg.drive, pair);
+  const uint32_t gu = (uint32_t)((ctx->n + kBlock - 1) / kBlock);
+  const uint32_t gp = prop_grid(ctx);
+  if (evs) (void)hipEventRecord(evs[0], ctx->stream);
+  hipLaunchKernelGGL(upd, dim3(gu), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+  if (evs) (void)hipEventRecord(evs[1], ctx->stream);
+  if (pair)
+    hipLaunchKernelGGL(k_propagate<true>, dim3(gp), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+  else
+    hipLaunchKernelGGL(k_propagate<false>, dim3(gp), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+  if (evs) (void)hipEventRecord(evs[2], ctx->stream);
+  if (pair) {
+    hipLaunchKernelGGL(k_ltd, dim3(gp), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+    if (evs) (void)hipEventRecord(evs[3], ctx->stream);
+  }
+  return hipGetLastError();
+}
+
+hipError_t enqueue_advance(nsx_ctx* ctx, int32_t g) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, ctx->stream, ctx->d.tbase, g);
+  return hipGetLastError();
+}
+
+}  // namespace nsx

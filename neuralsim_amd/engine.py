@@ -1,0 +1,275 @@
+"""Host API over the C ABI, shaped like the reference's engine and GPU backend.
+
+* `SimulationEngine`  -- src/simulation/engine.rs:43-823, device-resident (nsx_* entry points).
+* `GpuBackend`        -- src/simulation/gpu_backend.rs:47-1037, method for method
+                         (nsx_backend_* entry points).
+
+Everything runs through libnsx.so; there is no Python or CPU compute path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import check, load
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class SimulationEngine:
+    """Device-resident simulation of one network (SimulationEngine, engine.rs:43-80)."""
+
+    def __init__(self, cfg):
+        self._lib = load()
+        self.cfg = cfg.copy()
+        self.cfg.struct_size = C.sizeof(abi.NsxConfig)
+        h = C.c_void_p()
+        check(self._lib.nsx_create(C.byref(self.cfg), C.byref(h)), None, "nsx_create")
+        self._h = h
+        self.dt = float(cfg.dt_ms)
+
+    @classmethod
+    def from_spec(cls, spec):
+        eng = cls(spec.cfg)
+        eng.generate_fixed_indegree(*spec.generator_args())
+        return eng
+
+    # ---- lifetime -------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nsx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st, call):
+        return check(st, self._h, call)
+
+    # ---- network (Network::finalize, network/mod.rs:79-106) -------------------------------
+    def build_csr(self, src, dst, weight, delay):
+        src, dst = _c(src, np.uint32), _c(dst, np.uint32)
+        weight, delay = _c(weight, np.float32), _c(delay, np.uint8)
+        m = len(src)
+        assert len(dst) == m and len(weight) == m and len(delay) == m
+        self._chk(self._lib.nsx_build_csr(self._h, _ptr(src), _ptr(dst), _ptr(weight),
+                                          _ptr(delay), m), "nsx_build_csr")
+
+    finalize = build_csr
+
+    def upload_csr(self, row_ptr, col, weight, delay):
+        row_ptr, col = _c(row_ptr, np.uint64), _c(col, np.uint32)
+        weight, delay = _c(weight, np.float32), _c(delay, np.uint8)
+        n = len(row_ptr) - 1
+        self._chk(self._lib.nsx_upload_csr(self._h, _ptr(row_ptr), _ptr(col), _ptr(weight),
+                                           _ptr(delay), n, len(col)), "nsx_upload_csr")
+
+    def generate_fixed_indegree(self, k_in, w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_hi):
+        self._chk(self._lib.nsx_generate_fixed_indegree(
+            self._h, k_in, w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_hi),
+            "nsx_generate_fixed_indegree")
+
+    def size(self):
+        n, m = C.c_uint64(), C.c_uint64()
+        self._chk(self._lib.nsx_network_size(self._h, C.byref(n), C.byref(m)), "nsx_network_size")
+        return n.value, m.value
+
+    def read_csr(self):
+        n, m = self.size()
+        rp = np.empty(n + 1, np.uint64)
+        col = np.empty(m, np.uint32)
+        w = np.empty(m, np.float32)
+        d = np.empty(m, np.uint8)
+        self._chk(self._lib.nsx_read_csr(self._h, _ptr(rp), _ptr(col), _ptr(w), _ptr(d)),
+                  "nsx_read_csr")
+        return rp, col, w, d
+
+    def weights(self):
+        _, m = self.size()
+        w = np.empty(m, np.float32)
+        self._chk(self._lib.nsx_read_weights(self._h, _ptr(w), m), "nsx_read_weights")
+        return w
+
+    # ---- state ----------------------------------------------------------------------------
+    def read_state(self, fields=None, ring=False):
+        n = int(self.cfg.n_neurons)
+        fields = fields or list(abi.STATE_FIELDS)
+        out, view = {}, abi.NsxStateView()
+        for f in fields:
+            a = np.empty(n, abi.STATE_FIELDS[f])
+            out[f] = a
+            setattr(view, f, C.cast(a.ctypes.data, type(getattr(view, f))))
+        if ring:
+            r = np.empty(int(self.cfg.ring_slots) * abi.channels(self.cfg) * n, np.int64)
+            out["ring"] = r
+            view.ring = C.cast(r.ctypes.data, C.POINTER(C.c_int64))
+        self._chk(self._lib.nsx_read_state(self._h, C.byref(view)), "nsx_read_state")
+        return out
+
+    def upload_state(self, state):
+        view = abi.NsxStateView()
+        keep = []
+        for f, a in state.items():
+            if f == "ring":
+                a = _c(a, np.int64)
+                view.ring = C.cast(a.ctypes.data, C.POINTER(C.c_int64))
+            else:
+                a = _c(a, abi.STATE_FIELDS[f])
+                setattr(view, f, C.cast(a.ctypes.data, type(getattr(view, f))))
+            keep.append(a)
+        self._chk(self._lib.nsx_upload_state(self._h, C.byref(view)), "nsx_upload_state")
+
+    def reset_state(self):
+        self._chk(self._lib.nsx_reset_state(self._h), "nsx_reset_state")
+
+    def get_step(self):
+        t = C.c_uint64()
+        self._chk(self._lib.nsx_get_step(self._h, C.byref(t)), "nsx_get_step")
+        return t.value
+
+    def set_step(self, step):
+        self._chk(self._lib.nsx_set_step(self._h, int(step)), "nsx_set_step")
+
+    def stimulate(self, neuron_id, current):
+        """SimulationEngine::stimulate (engine.rs:156-161): input for the next step."""
+        self._chk(self._lib.nsx_stimulate(self._h, int(neuron_id), float(current)),
+                  "nsx_stimulate")
+
+    # ---- stepping (engine.rs:254-797) -------------------------------------------------------
+    def step(self, n_steps=1):
+        st = abi.NsxStepStats()
+        self._chk(self._lib.nsx_step(self._h, int(n_steps), C.byref(st)), "nsx_step")
+        return st
+
+    def simulate_ms(self, duration_ms):
+        st = abi.NsxStepStats()
+        self._chk(self._lib.nsx_simulate_ms(self._h, float(duration_ms), C.byref(st)),
+                  "nsx_simulate_ms")
+        return st
+
+    def set_graph_steps(self, g):
+        self._chk(self._lib.nsx_set_graph_steps(self._h, int(g)), "nsx_set_graph_steps")
+
+    def set_profiling(self, on):
+        self._chk(self._lib.nsx_set_profiling(self._h, 1 if on else 0), "nsx_set_profiling")
+
+    def kernel_time(self, kind):
+        ms, n, b = C.c_double(), C.c_uint64(), C.c_uint64()
+        self._chk(self._lib.nsx_kernel_time(self._h, kind, C.byref(ms), C.byref(n), C.byref(b)),
+                  "nsx_kernel_time")
+        return ms.value, n.value
+
+    def raster(self, clear=False):
+        navail = C.c_uint64()
+        self._chk(self._lib.nsx_read_raster(self._h, None, None, 0, C.byref(navail), 0),
+                  "nsx_read_raster")
+        k = navail.value
+        steps = np.empty(k, np.uint32)
+        ids = np.empty(k, np.uint32)
+        self._chk(self._lib.nsx_read_raster(self._h, _ptr(steps), _ptr(ids), k,
+                                            C.byref(navail), 1 if clear else 0),
+                  "nsx_read_raster")
+        return steps, ids
+
+    def step_counts(self, cap=1 << 20):
+        out = np.empty(cap, np.uint32)
+        n = C.c_uint64()
+        self._chk(self._lib.nsx_read_step_counts(self._h, _ptr(out), cap, C.byref(n)),
+                  "nsx_read_step_counts")
+        return out[: min(n.value, cap)]
+
+    def stats(self):
+        """SimulationStats subset (engine.rs:21-33)."""
+        st = self.step(0)
+        return {"total_spikes": st.total_spikes, "total_events": st.total_events,
+                "sim_time_ms": st.sim_time_ms}
+
+
+class GpuBackend:
+    """Drop-in for the wgpu GpuBackend (gpu_backend.rs:47-95), same methods and semantics."""
+
+    LIF, IZHIKEVICH, HODGKIN_HUXLEY = 0, 1, 2
+
+    def __init__(self, num_neurons, num_edges):
+        """GpuBackend::new (gpu_backend.rs:98)."""
+        self._lib = load()
+        h = C.c_void_p()
+        check(self._lib.nsx_backend_new(int(num_neurons), int(num_edges), C.byref(h)), None,
+              "nsx_backend_new")
+        self._h = h
+        self.n = int(num_neurons)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nsx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st, call):
+        return check(st, self._h, call)
+
+    def upload_initial_state(self, membrane_potential, recovery_variable, refractory_counter,
+                             last_spike_time, spike_count, params, is_output):
+        a = [_c(membrane_potential, np.float32), _c(recovery_variable, np.float32),
+             _c(refractory_counter, np.int32), _c(last_spike_time, np.float32),
+             _c(spike_count, np.uint32), _c(params, np.float32).reshape(-1),
+             _c(is_output, np.uint32)]
+        self._chk(self._lib.nsx_backend_upload_initial_state(self._h, *[_ptr(x) for x in a]),
+                  "upload_initial_state")
+
+    def upload_hh_gates(self, m, h, n):
+        a = [_c(m, np.float32), _c(h, np.float32), _c(n, np.float32)]
+        self._chk(self._lib.nsx_backend_upload_hh_gates(self._h, *[_ptr(x) for x in a]),
+                  "upload_hh_gates")
+
+    def upload_csr(self, adjacency_ptr, adjacency_indices, synapse_weights):
+        a = [_c(adjacency_ptr, np.uint32), _c(adjacency_indices, np.uint32),
+             _c(synapse_weights, np.float32)]
+        self._chk(self._lib.nsx_backend_upload_csr(self._h, *[_ptr(x) for x in a]), "upload_csr")
+
+    def upload_input_current(self, input_current):
+        a = _c(input_current, np.float32)
+        self._chk(self._lib.nsx_backend_upload_input_current(self._h, _ptr(a)),
+                  "upload_input_current")
+
+    def write_uniforms(self, dt, sim_time):
+        self._chk(self._lib.nsx_backend_write_uniforms(self._h, float(dt), float(sim_time)),
+                  "write_uniforms")
+
+    def set_model(self, model):
+        self._chk(self._lib.nsx_backend_set_model(self._h, int(model)), "set_model")
+
+    def step_neurons(self):
+        n = self.n
+        sp, v, u = np.empty(n, np.uint32), np.empty(n, np.float32), np.empty(n, np.float32)
+        refr, cnt = np.empty(n, np.int32), np.empty(n, np.uint32)
+        self._chk(self._lib.nsx_backend_step_neurons(self._h, _ptr(sp), _ptr(v), _ptr(u),
+                                                     _ptr(refr), _ptr(cnt)), "step_neurons")
+        return sp, v, u, refr, cnt
+
+    step_lif = step_neurons
+
+    def step_spmv(self):
+        out = np.empty(self.n, np.float32)
+        self._chk(self._lib.nsx_backend_step_spmv(self._h, _ptr(out)), "step_spmv")
+        return out

@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--graph", type=int, default=0, help="steps per captured hipGraph (0: off)")
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
+    ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -92,6 +93,8 @@ def main():
     log(f"bench: {spec.name} n={spec.n} m={spec.m} built on device in {build_s:.2f}s")
     if args.graph:
         eng.set_graph_steps(args.graph)
+    if args.split:
+        eng.set_prop_split(args.split)
     eng.step(args.warmup)
     log(f"bench: warmup {args.warmup} steps done")
     prof = not args.no_profile and not args.graph
@@ -129,18 +132,33 @@ def main():
         "build_s": build_s,
     }
     if prof:
-        kinds = {0: "k_update", 1: "k_propagate", 2: "k_ltd"}
+        kinds = {0: "k_update", 1: "k_synapses"}
         times = {k: eng.kernel_time(k) for k in kinds}
-        upd_b = neuron_bytes(spec.cfg, spec.n, spikes / args.steps)
-        prop_b = (17.0 * events + 16.0 * spikes) / args.steps
+        ltp_slots, ltd_slots = times[1][2], eng.kernel_time(2)[2]
+        per = 1.0 / args.steps
+        upd_b = neuron_bytes(spec.cfg, spec.n, spikes * per)
+        # propagation: 4 B col + 4 B weight + 1 B delay + 8 B int64 accumulate per event and
+        # 16 B of row pointers per spiking row; potentiation: 16 B post record + 4 B weight
+        # write per E->E out-slot; depression: 4 B slot + 4 B source + 8 B pre record + 8 B
+        # weight read+write per E->E in-slot
+        prop_b = (17.0 * events + 16.0 * spikes) * per
+        syn_b = prop_b + (20.0 * ltp_slots + 24.0 * ltd_slots) * per
         kt = {kinds[k]: {"avg_us": (ms / max(n, 1)) * 1e3, "launches": n}
-              for k, (ms, n) in times.items()}
+              for k, (ms, n, _) in times.items()}
         kt["k_update"]["alg_bytes_per_launch"] = upd_b
-        kt["k_propagate"]["alg_bytes_per_launch"] = prop_b
-        for k in ("k_update", "k_propagate"):
-            a = kt[k]["alg_bytes_per_launch"] / (kt[k]["avg_us"] * 1e-6) / 1e9
+        kt["k_synapses"]["alg_bytes_per_launch"] = syn_b
+        kt["k_synapses"]["propagation_bytes_per_launch"] = prop_b
+        kt["k_synapses"]["events_per_launch"] = events * per
+        kt["k_synapses"]["ltp_slots_per_launch"] = ltp_slots * per
+        kt["k_synapses"]["ltd_slots_per_launch"] = ltd_slots * per
+        for k in kinds.values():
+            a = kt[k]["alg_bytes_per_launch"] / (max(kt[k]["avg_us"], 1e-9) * 1e-6) / 1e9
             kt[k]["achieved_GBs"] = a
-        dom = max(("k_update", "k_propagate"), key=lambda k: kt[k]["avg_us"])
+        # the synapse pass issues one 64-bit atomic per event: rate vs the measured random
+        # atomic throughput of the device (23.0 G/s, int64 add to random addresses)
+        kt["k_synapses"]["atomics_G_per_s"] = events * per / (kt["k_synapses"]["avg_us"] * 1e3)
+        kt["k_synapses"]["atomic_ceiling_G_per_s"] = 23.0
+        dom = max(kinds.values(), key=lambda k: kt[k]["avg_us"])
         out["roofline"] = {
             "kernel": dom, "bound": "hbm", "achieved": kt[dom]["achieved_GBs"],
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kt[dom]["achieved_GBs"] / HBM_PEAK_GBS,

@@ -209,6 +209,9 @@ nsx_status nsx_stimulate(nsx_ctx* ctx, uint64_t neuron, float current);
 /* Advance n_steps steps (engine.rs:254-680, device-resident).  stats may be NULL. */
 nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* stats);
 nsx_status nsx_simulate_ms(nsx_ctx* ctx, double duration_ms, nsx_step_stats* stats);
+/* Launch shape: wavefronts that share one 256-neuron block's spiking rows in propagation and
+ * depression (1..64; default 4).  Results do not depend on it. */
+nsx_status nsx_set_prop_split(nsx_ctx* ctx, uint32_t waves_per_block);
 /* Use a captured hipGraph of `steps_per_graph` steps for nsx_step (0 = direct launches). */
 nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph);
 

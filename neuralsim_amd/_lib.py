@@ -52,6 +52,7 @@ PROTOTYPES = {
     "nsx_step": (_int, [_VP, _u64, C.POINTER(abi.NsxStepStats)]),
     "nsx_simulate_ms": (_int, [_VP, _f64, C.POINTER(abi.NsxStepStats)]),
     "nsx_set_graph_steps": (_int, [_VP, _u32]),
+    "nsx_set_prop_split": (_int, [_VP, _u32]),
     "nsx_read_raster": (_int, [_VP, _P, _P, _u64, C.POINTER(_u64), _int]),
     "nsx_read_step_counts": (_int, [_VP, _P, _u64, C.POINTER(_u64)]),
     "nsx_set_profiling": (_int, [_VP, _int]),

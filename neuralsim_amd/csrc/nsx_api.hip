@@ -68,10 +68,8 @@ __global__ void k_init_state(StepParams p, DevPtrs s, int model, nsx_config cfg)
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
   s.last[i] = kNeverStep;
-  s.pre_s[i] = kNeverStep;
-  s.post_s[i] = kNeverStep;
-  s.pre_v[i] = 0.0f;
-  s.post_v[i] = 0.0f;
+  s.pre[i] = TraceRec{0.0f, kNeverStep};
+  s.post[i] = PostRec{0.0f, kNeverStep, kNeverStep, 0};
   s.count[i] = 0u;
   s.refr[i] = 0;
   s.s0[i] = 0.0f;
@@ -163,6 +161,7 @@ static void fill_params(nsx_ctx* ctx) {
   p.n = (uint32_t)c.n_neurons;
   p.n_exc = (uint32_t)c.n_exc;
   p.nblk = (uint32_t)((c.n_neurons + kBlock - 1) / kBlock);
+  p.prop_split = 4;
   p.D = c.ring_slots;
   p.ch = (uint32_t)ctx->ch;
   p.seed = c.seed;
@@ -221,10 +220,12 @@ static void fill_params(nsx_ctx* ctx) {
   p.raster_cap = c.record_raster ? c.raster_capacity : 0;
 }
 
-static nsx_status make_trace_table(nsx_ctx* ctx, double tau, float** out, int32_t* len) {
+// Lazy trace decay table P[k] = exp(-k dt / tau) rounded to float; entries past the end read 0.
+static nsx_status make_trace_table(nsx_ctx* ctx, double tau, float** out, int32_t* len,
+                                   std::vector<float>& h) {
   const double dt = ctx->cfg.dt_ms;
   int n = (int)ceil(104.0 * tau / dt) + 2;
-  std::vector<float> h((size_t)n);
+  h.assign((size_t)n, 0.0f);
   for (int k = 0; k < n; ++k) h[(size_t)k] = (float)exp(-(double)k * dt / tau);
   nsx_status s = dev_alloc(ctx, (void**)out, (size_t)n * 4, false);
   if (s) return s;
@@ -377,14 +378,14 @@ nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out) {
   const size_t n = ctx->n;
   DevPtrs& d = ctx->d;
   float** f32s[] = {&d.v, &d.u, &d.gm, &d.gh, &d.gn, &d.s0, &d.s1, &d.s2, &d.drv, &d.c, &d.d,
-                    &d.pre_v, &d.post_v};
+                   };
   for (auto pp : f32s)
     if ((s = dev_alloc(ctx, (void**)pp, n * 4, true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.refr, n * 4, true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.last, n * 4, true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.count, n * 4, true))) goto fail;
-  if ((s = dev_alloc(ctx, (void**)&d.pre_s, n * 4, true))) goto fail;
-  if ((s = dev_alloc(ctx, (void**)&d.post_s, n * 4, true))) goto fail;
+  if ((s = dev_alloc(ctx, (void**)&d.pre, n * 8, true))) goto fail;
+  if ((s = dev_alloc(ctx, (void**)&d.post, n * sizeof(PostRec), true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.ring, (size_t)ctx->D * ctx->ch * n * 8, true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.spikes, (size_t)ctx->p.nblk * kBlock * 4, false))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.bcnt, (size_t)ctx->p.nblk * 4, true))) goto fail;
@@ -401,8 +402,8 @@ nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out) {
   if (cfg->plasticity == NSX_PLAST_PAIR) {
     float* pt = nullptr;
     float* mt = nullptr;
-    if ((s = make_trace_table(ctx, cfg->tau_plus_ms, &pt, &ctx->p.ptab_n))) goto fail;
-    if ((s = make_trace_table(ctx, cfg->tau_minus_ms, &mt, &ctx->p.mtab_n))) goto fail;
+    if ((s = make_trace_table(ctx, cfg->tau_plus_ms, &pt, &ctx->p.ptab_n, ctx->ptab_h))) goto fail;
+    if ((s = make_trace_table(ctx, cfg->tau_minus_ms, &mt, &ctx->p.mtab_n, ctx->mtab_h))) goto fail;
     d.ptab = pt;
     d.mtab = mt;
   }
@@ -459,6 +460,80 @@ static nsx_status copy_f(nsx_ctx* ctx, void* dst, const void* src, size_t bytes,
   return NSX_OK;
 }
 
+// Trace records are packed {value, step} on the device; the state view splits them.
+static nsx_status put_traces(nsx_ctx* ctx, TraceRec* dst, const float* v, const int32_t* st) {
+  if (!v && !st) return NSX_OK;
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<TraceRec> h(ctx->n);
+  NSX_HIP(ctx, hipMemcpy(h.data(), dst, ctx->n * 8, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < ctx->n; ++i) {
+    if (v) h[i].v = v[i];
+    if (st) h[i].s = st[i];
+  }
+  NSX_HIP(ctx, hipMemcpy(dst, h.data(), ctx->n * 8, hipMemcpyHostToDevice));
+  return NSX_OK;
+}
+
+static nsx_status get_traces(nsx_ctx* ctx, const TraceRec* src, float* v, int32_t* st) {
+  if (!v && !st) return NSX_OK;
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<TraceRec> h(ctx->n);
+  NSX_HIP(ctx, hipMemcpy(h.data(), src, ctx->n * 8, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < ctx->n; ++i) {
+    if (v) v[i] = h[i].v;
+    if (st) st[i] = h[i].s;
+  }
+  return NSX_OK;
+}
+
+// Post records: uploading reference-view traces leaves no bump pending.
+static nsx_status put_post(nsx_ctx* ctx, const float* v, const int32_t* st, const int32_t* last) {
+  if (!v && !st) return NSX_OK;
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<PostRec> h(ctx->n);
+  NSX_HIP(ctx, hipMemcpy(h.data(), ctx->d.post, ctx->n * sizeof(PostRec), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < ctx->n; ++i) {
+    if (v) h[i].v = v[i];
+    if (st) h[i].s = st[i];
+    h[i].last = last ? last[i] : kNeverStep;
+  }
+  NSX_HIP(ctx, hipMemcpy(ctx->d.post, h.data(), ctx->n * sizeof(PostRec), hipMemcpyHostToDevice));
+  NSX_HIP(ctx, hipMemset(ctx->d.bcnt, 0, (size_t)ctx->p.nblk * 4));
+  return NSX_OK;
+}
+
+// Post records in the reference's view: the bumps of the last step's spikes, which the device
+// applies at the start of the next step, are applied here with the same float operations.
+static nsx_status get_post(nsx_ctx* ctx, float* v, int32_t* st) {
+  if (!v && !st) return NSX_OK;
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const size_t n = ctx->n, nb = ctx->p.nblk;
+  std::vector<PostRec> h(n);
+  NSX_HIP(ctx, hipMemcpy(h.data(), ctx->d.post, n * sizeof(PostRec), hipMemcpyDeviceToHost));
+  if (ctx->cfg.plasticity == NSX_PLAST_PAIR && ctx->t > 0) {
+    std::vector<uint32_t> cnt(nb), seg(nb * kBlock);
+    NSX_HIP(ctx, hipMemcpy(cnt.data(), ctx->d.bcnt, nb * 4, hipMemcpyDeviceToHost));
+    NSX_HIP(ctx, hipMemcpy(seg.data(), ctx->d.spikes, nb * kBlock * 4, hipMemcpyDeviceToHost));
+    const int32_t t = (int32_t)ctx->t - 1;
+    for (size_t b = 0; b < nb; ++b)
+      for (uint32_t k = 0; k < cnt[b]; ++k) {
+        const uint32_t j = seg[b * kBlock + k];
+        if (j >= ctx->cfg.n_exc) continue;
+        const int32_t dk = t - h[j].s;
+        const float q = (dk >= 0 && dk < ctx->p.mtab_n) ? ctx->mtab_h[(size_t)dk] : 0.0f;
+        float x = h[j].v * q;
+        x = x * q;
+        h[j].v = x + 1.0f;
+        h[j].s = t;
+      }
+  }
+  for (size_t i = 0; i < n; ++i) {
+    if (v) v[i] = h[i].v;
+    if (st) st[i] = h[i].s;
+  }
+  return NSX_OK;
+}
+
 nsx_status nsx_upload_state(nsx_ctx* ctx, const nsx_state_view* sv) {
   if (!ctx || !sv) return set_error(ctx, NSX_E_INVALID, "NULL argument");
   const size_t b = ctx->n * 4;
@@ -472,10 +547,8 @@ nsx_status nsx_upload_state(nsx_ctx* ctx, const nsx_state_view* sv) {
       (s = copy_f(ctx, d.drv, sv->drive, b, k)) || (s = copy_f(ctx, d.refr, sv->refractory, b, k)) ||
       (s = copy_f(ctx, d.last, sv->last_spike_step, b, k)) ||
       (s = copy_f(ctx, d.count, sv->spike_count, b, k)) ||
-      (s = copy_f(ctx, d.pre_v, sv->pre_trace, b, k)) ||
-      (s = copy_f(ctx, d.pre_s, sv->pre_step, b, k)) ||
-      (s = copy_f(ctx, d.post_v, sv->post_trace, b, k)) ||
-      (s = copy_f(ctx, d.post_s, sv->post_step, b, k)) ||
+      (s = put_traces(ctx, d.pre, sv->pre_trace, sv->pre_step)) ||
+      (s = put_post(ctx, sv->post_trace, sv->post_step, sv->last_spike_step)) ||
       (s = copy_f(ctx, d.ring, sv->ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)))
     return s;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -495,10 +568,8 @@ nsx_status nsx_read_state(nsx_ctx* ctx, nsx_state_view* sv) {
       (s = copy_f(ctx, sv->drive, d.drv, b, k)) || (s = copy_f(ctx, sv->refractory, d.refr, b, k)) ||
       (s = copy_f(ctx, sv->last_spike_step, d.last, b, k)) ||
       (s = copy_f(ctx, sv->spike_count, d.count, b, k)) ||
-      (s = copy_f(ctx, sv->pre_trace, d.pre_v, b, k)) ||
-      (s = copy_f(ctx, sv->pre_step, d.pre_s, b, k)) ||
-      (s = copy_f(ctx, sv->post_trace, d.post_v, b, k)) ||
-      (s = copy_f(ctx, sv->post_step, d.post_s, b, k)) ||
+      (s = get_traces(ctx, d.pre, sv->pre_trace, sv->pre_step)) ||
+      (s = get_post(ctx, sv->post_trace, sv->post_step)) ||
       (s = copy_f(ctx, sv->ring, d.ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)))
     return s;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -534,10 +605,27 @@ nsx_status nsx_get_step(nsx_ctx* ctx, uint64_t* step) {
 nsx_status nsx_set_step(nsx_ctx* ctx, uint64_t step) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (step >= (1ull << 30)) return set_error(ctx, NSX_E_INVALID, "step index limit");
-  // the per-step spike counter of the next step must read zero
+  // post trace bumps pending from the last step are keyed by that step: apply them now
+  if (ctx->cfg.plasticity == NSX_PLAST_PAIR && ctx->t > 0) {
+    std::vector<float> v(ctx->n);
+    std::vector<int32_t> st(ctx->n), last(ctx->n);
+    nsx_status s = get_post(ctx, v.data(), st.data());
+    if (s) return s;
+    NSX_HIP(ctx, hipMemcpy(last.data(), ctx->d.last, ctx->n * 4, hipMemcpyDeviceToHost));
+    if ((s = put_post(ctx, v.data(), st.data(), last.data()))) return s;
+  }
   NSX_HIP(ctx, hipMemsetAsync(ctx->d.nspk, 0, 16, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->t = step;
+  return NSX_OK;
+}
+
+nsx_status nsx_set_prop_split(nsx_ctx* ctx, uint32_t waves_per_block) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (waves_per_block < 1 || waves_per_block > 64)
+    return set_error(ctx, NSX_E_INVALID, "waves_per_block must be in [1, 64]");
+  ctx->p.prop_split = waves_per_block;
+  destroy_graph(ctx);  // captured launches hold the old shape
   return NSX_OK;
 }
 
@@ -613,7 +701,8 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   NSX_HIP(ctx, hipEventRecord(e_beg, ctx->stream));
   const bool prof = ctx->profiling;
   const bool pair = ctx->cfg.plasticity == NSX_PLAST_PAIR;
-  const size_t nev = pair ? 4 : 3;
+  const size_t nev = 3;  // events around k_update and k_synapses
+  (void)pair;
   if (prof && ctx->ev_pool.size() < nev * n_steps) {
     size_t have = ctx->ev_pool.size();
     ctx->ev_pool.resize(nev * n_steps);

@@ -14,10 +14,28 @@ namespace nsx {
 constexpr int kMaxPoisson = 32;
 constexpr uint32_t kBlock = 256;
 
+// Lazily decayed STDP trace of one neuron: value right after its last bump and that step.
+struct __attribute__((aligned(8))) TraceRec {
+  float v;
+  int32_t s;
+};
+
+// Post-synaptic trace record: the lazily decayed trace, plus the step of the neuron's last spike
+// so that the potentiation pass sees same-step post spikes in the same cache line.  The trace
+// bump of a spike at step t is applied by the update kernel of step t+1 (after every pass of
+// step t has read the pre-bump value).
+struct __attribute__((aligned(16))) PostRec {
+  float v;
+  int32_t s;
+  int32_t last;
+  int32_t pad;
+};
+
 // Everything a step kernel reads that is constant over a run (passed by value).
 struct StepParams {
   uint32_t n, n_exc, D, ch;
-  uint32_t nblk;  // neuron blocks of kBlock neurons (spike segments)
+  uint32_t nblk;        // neuron blocks of kBlock neurons (spike segments)
+  uint32_t prop_split;  // wavefronts sharing one block's spiking rows (propagation / LTD)
   uint64_t seed;
   double scale_in, scale_out;  // 2^S, 2^-S
   // LIF
@@ -50,8 +68,8 @@ struct DevPtrs {
   float *v, *u, *gm, *gh, *gn, *s0, *s1, *s2, *drv, *c, *d;
   int32_t *refr, *last;
   uint32_t* count;
-  float *pre_v, *post_v;
-  int32_t *pre_s, *post_s;
+  TraceRec* pre;
+  PostRec* post;
   long long* ring;
   const float *ptab, *mtab;
   uint32_t* spikes;  // per-block spike segments: block b's spikes at [b*kBlock, +bcnt[b])
@@ -111,6 +129,7 @@ struct nsx_ctx {
   uint32_t* csc_slot = nullptr;
   uint32_t* csc_src = nullptr;
   uint64_t csc_m = 0;
+  std::vector<float> ptab_h, mtab_h;  // host copies of the trace decay tables
   // graph
   uint32_t graph_steps = 0;
   hipGraphExec_t graph_exec = nullptr;

@@ -1,17 +1,21 @@
 // nsx_step.hip -- per-step kernels of the simulation core (gfx950, wave64).
 //
-// One step t is three launches on one stream:
+// One step t is two launches on one stream:
 //   k_update     one thread per neuron: read+clear ring slot t, external drive, model update
-//                (LIF / Izhikevich / HH), spike compaction (wave ballot -> one atomic per wave),
-//                pre-synaptic trace bump.             Replaces SimulationEngine::step phase 1
+//                (LIF / Izhikevich / HH), block-local spike compaction into the block's spike
+//                segment, pre-synaptic trace bump; first applies the post-synaptic trace bumps
+//                of the block's spikes of step t-1.  Replaces SimulationEngine::step phase 1
 //                (engine.rs:267-391) and the lif/izhikevich/hh_update.wgsl shaders.
-//   k_propagate  one wavefront per spiking row: coalesced col/weight/delay streams, int64
-//                fixed-point atomic add into ring[(t+d) mod D][channel][target]; pair-STDP
-//                potentiation of E->E synapses of the row in the same pass.  Replaces the
-//                propagation loop engine.rs:414-473 (delays honoured) and spmv.wgsl.
-//   k_ltd        one wavefront per spiking excitatory neuron: depression of its E->E in-synapses
-//                through the transpose index, then its post-synaptic trace bump.  Together with
-//                k_propagate this is engine.rs:496-537 with per-neuron traces.
+//   k_synapses   two kinds of wavefront work items in one grid, interleaved:
+//                * propagation: a neuron block's spiking rows flattened across the 64 lanes
+//                  (coalesced col / weight / delay streams); each slot adds rint(w 2^S) into
+//                  ring[(t+d) mod D][channel][target] with a 64-bit integer atomic; E->E slots
+//                  get pair-STDP potentiation, then depression when the target fired at t too
+//                  (engine.rs:414-473 with delays, engine.rs:512-534; spmv.wgsl);
+//                * depression: E->E in-slots (transpose index) of a block's spiking excitatory
+//                  neurons whose source did not fire at t (engine.rs:524-534).
+//                Every weight has one writer per step, so the kinds run concurrently: the
+//                atomic-throughput-bound propagation overlaps the latency-bound depression.
 // Kernels read the step index as *tbase + off so that a captured hipGraph of G steps can be
 // replayed (k_advance moves tbase by G at the end of the graph).
 #include "nsx_device.h"
@@ -21,6 +25,31 @@ namespace nsx {
 
 __device__ inline float trace_at(const float* tab, int32_t tab_n, int32_t k) {
   return (k >= 0 && k < tab_n) ? tab[k] : 0.0f;
+}
+
+// Trace value at step t of a record last bumped at step s (lazy decay through the table).
+__device__ inline float trace_now(float v, int32_t s, const float* tab, int32_t tab_n,
+                                  int32_t t) {
+  const float q = trace_at(tab, tab_n, t - s);
+  return v * q;
+}
+
+// StdpTrace::on_pre_spike / on_post_spike (plasticity.rs:144-154) on a lazily decayed record:
+// the per-step decay and the elapsed-time factor are both P[t - s].
+__device__ inline TraceRec trace_bump(const TraceRec rec, const float* tab, int32_t tab_n,
+                                      int32_t t) {
+  const float q = trace_at(tab, tab_n, t - rec.s);
+  float x = rec.v * q;
+  x = x * q;
+  return TraceRec{x + 1.0f, t};
+}
+
+__device__ inline PostRec post_bump(const PostRec rec, const float* tab, int32_t tab_n,
+                                    int32_t t) {
+  const float q = trace_at(tab, tab_n, t - rec.s);
+  float x = rec.v * q;
+  x = x * q;
+  return PostRec{x + 1.0f, t, rec.last, 0};
 }
 
 // Per-step drive value for neuron i (value added to channel 0 / the input current).
@@ -56,6 +85,15 @@ template <int MODEL, int DRIVE, bool PAIR>
 __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
   const int32_t t = *s.tbase + off;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (PAIR) {
+    // post trace bumps of this block's spikes at step t-1 (its segment still holds them)
+    const uint32_t prev = s.bcnt[blockIdx.x];
+    if (threadIdx.x < prev) {
+      const uint32_t j = s.spikes[blockIdx.x * kBlock + threadIdx.x];
+      if (j < p.n_exc) s.post[j] = post_bump(s.post[j], s.mtab, p.mtab_n, t - 1);
+    }
+    __syncthreads();  // the segment is rewritten below
+  }
   bool spiked = false;
   if (i < p.n) {
     const uint32_t slot = (uint32_t)t % p.D;
@@ -221,12 +259,8 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       s.count[i] += 1u;
       s.last[i] = t;
       if (PAIR && i < p.n_exc) {
-        const int32_t ps = s.pre_s[i];
-        float q = trace_at(s.ptab, p.ptab_n, t - ps);
-        float x = s.pre_v[i] * q;
-        x = x * q;
-        s.pre_v[i] = x + 1.0f;
-        s.pre_s[i] = t;
+        s.pre[i] = trace_bump(s.pre[i], s.ptab, p.ptab_n, t);
+        s.post[i].last = t;
       }
     }
   }
@@ -285,140 +319,216 @@ struct RowChunk {
   uint64_t beg[64];
 };
 
-// One wavefront per neuron block: all spiking rows of the block's segment, flattened so
-// that every lane streams slots (coalesced col / weight / delay), int64 fixed-point atomic
-// add into the ring slot of the arrival step, pair-STDP potentiation on E->E slots.
+// Inclusive wave scan of deg; stores prefix / start / source of each lane's row in LDS and
+// returns the total.
+__device__ inline uint32_t chunk_setup(RowChunk& rc, uint32_t lane, uint32_t deg, uint64_t beg,
+                                       uint32_t src) {
+  uint32_t incl = deg;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  rc.incl[lane] = incl;
+  rc.src[lane] = src;
+  rc.beg[lane] = beg;
+  wave_sync();
+  return __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+}
+
+// Slot position of flat index f (rows visited in increasing order by each lane).
+__device__ inline uint64_t chunk_slot(const RowChunk& rc, uint32_t f, uint32_t& r) {
+  while (rc.incl[r] <= f) ++r;
+  const uint32_t excl = r ? rc.incl[r - 1] : 0u;
+  return rc.beg[r] + (f - excl);
+}
+
+// Propagation work item: part q of neuron block b.
 template <bool PAIR>
-__global__ __launch_bounds__(kBlock) void k_propagate(StepParams p, DevPtrs s, int32_t off) {
-  __shared__ RowChunk sh[kBlock / 64];
-  const int32_t t = *s.tbase + off;
-  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
-  const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+__device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
+                               uint32_t b, uint32_t q, uint32_t lane) {
+  const uint32_t P = p.prop_split;
+  const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
+  if (cnt <= q) return;
+  const uint32_t mine = (cnt - q + P - 1) / P;  // rows q, q+P, ... of the segment
   const uint32_t tm = (uint32_t)t % p.D;
   const size_t plane = (size_t)p.ch * p.n;
-  RowChunk& rc = sh[wid];
-  for (uint32_t b = wave; b < p.nblk; b += nwaves) {
-    const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
-    if (cnt == 0) continue;
-    unsigned long long ev = 0, ltp = 0;
-    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
-      const uint32_t nr = min(64u, cnt - c0);
-      uint32_t i = 0, deg = 0;
-      uint64_t beg = 0;
-      if (lane < nr) {
-        i = s.spikes[b * kBlock + c0 + lane];
-        beg = s.row_ptr[i];
-        deg = (uint32_t)(s.row_ptr[i + 1] - beg);
-      }
-      uint32_t incl = deg;
-#pragma unroll
-      for (uint32_t o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-      }
-      const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
-      rc.incl[lane] = incl;
-      rc.src[lane] = i;
-      rc.beg[lane] = beg;
-      wave_sync();
-      ev += total;
-      uint32_t r = 0;
-      for (uint32_t f = lane; f < total; f += 64) {
-        while (rc.incl[r] <= f) ++r;
-        const uint32_t excl = r ? rc.incl[r - 1] : 0u;
-        const uint64_t k = rc.beg[r] + (f - excl);
-        const uint32_t src = rc.src[r];
-        const uint32_t j = s.col[k];
-        const float w = s.w[k];
-        uint32_t sl = tm + (uint32_t)s.delay[k];
-        sl = sl >= p.D ? sl - p.D : sl;
-        const uint32_t chn = (p.ch > 1 && src >= p.n_exc) ? 1u : 0u;
-        atomicAdd((unsigned long long*)(s.ring + sl * plane + (size_t)chn * p.n + j),
-                  (unsigned long long)quantize(w, p.scale_in));
-        if (PAIR && src < p.n_exc && j < p.n_exc) {
-          const float q = trace_at(s.mtab, p.mtab_n, t - s.post_s[j]);
-          const float x = s.post_v[j] * q;
+  unsigned long long ev = 0, ltp = 0;
+  for (uint32_t c0 = 0; c0 < mine; c0 += 64) {
+    const uint32_t nr = min(64u, mine - c0);
+    uint32_t i = 0, deg = 0;
+    uint64_t beg = 0;
+    if (lane < nr) {
+      i = s.spikes[b * kBlock + q + (c0 + lane) * P];
+      beg = s.row_ptr[i];
+      deg = (uint32_t)(s.row_ptr[i + 1] - beg);
+    }
+    const uint32_t total = chunk_setup(rc, lane, deg, beg, i);
+    ev += total;
+    uint32_t r = 0;
+    for (uint32_t f = lane; f < total; f += 128) {
+      // two slots per lane: both sets of loads are issued before the atomics
+      const bool two = f + 64 < total;
+      const uint64_t k0 = chunk_slot(rc, f, r);
+      const uint32_t src0 = rc.src[r];
+      uint32_t r1 = r;
+      const uint64_t k1 = two ? chunk_slot(rc, f + 64, r1) : k0;
+      const uint32_t src1 = rc.src[r1];
+      const uint32_t j0 = s.col[k0], j1 = s.col[k1];
+      const float w0 = s.w[k0], w1 = s.w[k1];
+      const uint32_t d0 = s.delay[k0], d1 = s.delay[k1];
+      const bool pot0 = PAIR && src0 < p.n_exc && j0 < p.n_exc;
+      const bool pot1 = PAIR && two && src1 < p.n_exc && j1 < p.n_exc;
+      PostRec pr0{0.0f, kNeverStep, kNeverStep, 0}, pr1{0.0f, kNeverStep, kNeverStep, 0};
+      if (pot0) pr0 = s.post[j0];
+      if (pot1) pr1 = s.post[j1];
+      uint32_t sl0 = tm + d0, sl1 = tm + d1;
+      sl0 = sl0 >= p.D ? sl0 - p.D : sl0;
+      sl1 = sl1 >= p.D ? sl1 - p.D : sl1;
+      const uint32_t ch0 = (p.ch > 1 && src0 >= p.n_exc) ? 1u : 0u;
+      const uint32_t ch1 = (p.ch > 1 && src1 >= p.n_exc) ? 1u : 0u;
+      atomicAdd((unsigned long long*)(s.ring + sl0 * plane + (size_t)ch0 * p.n + j0),
+                (unsigned long long)quantize(w0, p.scale_in));
+      if (two)
+        atomicAdd((unsigned long long*)(s.ring + sl1 * plane + (size_t)ch1 * p.n + j1),
+                  (unsigned long long)quantize(w1, p.scale_in));
+      if (PAIR && (pot0 || pot1)) {
+        // the source fired at t, so its pre trace is the record bumped at t (factor P[0] = 1)
+        if (pot0) {
+          const float x = trace_now(pr0.v, pr0.s, s.mtab, p.mtab_n, t);
           const float dlt = p.a_plus * x;
-          const float wn = w + dlt;
-          s.w[k] = clampf_(wn, p.w_min, p.w_max);
+          float wn = w0 + dlt;
+          wn = clampf_(wn, p.w_min, p.w_max);
+          if (pr0.last == t) {  // target fired at t as well: depression follows
+            const float y = p.a_minus * s.pre[src0].v;
+            wn = wn - y;
+            wn = clampf_(wn, p.w_min, p.w_max);
+          }
+          s.w[k0] = wn;
+          ltp += 1;
+        }
+        if (pot1) {
+          const float x = trace_now(pr1.v, pr1.s, s.mtab, p.mtab_n, t);
+          const float dlt = p.a_plus * x;
+          float wn = w1 + dlt;
+          wn = clampf_(wn, p.w_min, p.w_max);
+          if (pr1.last == t) {
+            const float y = p.a_minus * s.pre[src1].v;
+            wn = wn - y;
+            wn = clampf_(wn, p.w_min, p.w_max);
+          }
+          s.w[k1] = wn;
           ltp += 1;
         }
       }
-      wave_sync();
+      r = r1;
     }
-    if (PAIR) {
+    wave_sync();
+  }
+  if (PAIR) {
 #pragma unroll
-      for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
-    }
-    if (lane == 0) {
+    for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
+  }
+  if (lane == 0) {
+    // parts of one block accumulate into that block's counter
+    if (P == 1) {
       s.part[kPartEvents * p.nblk + b] += ev;
       if (PAIR) s.part[kPartLtp * p.nblk + b] += ltp;
+    } else {
+      atomicAdd(&s.part[kPartEvents * p.nblk + b], ev);
+      if (PAIR) atomicAdd(&s.part[kPartLtp * p.nblk + b], ltp);
     }
   }
 }
 
-// Post-spike depression: one wavefront per neuron block, the E->E in-slots (transpose index)
-// of the block's spiking excitatory neurons flattened across lanes; then their post trace bump.
-__global__ __launch_bounds__(kBlock) void k_ltd(StepParams p, DevPtrs s, int32_t off) {
+// Depression work item: part q of neuron block b (excitatory neurons only).
+__device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
+                             uint32_t b, uint32_t q, uint32_t lane) {
+  const uint32_t P = p.prop_split;
+  const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
+  if (cnt <= q) return;
+  const uint32_t mine = (cnt - q + P - 1) / P;
+  unsigned long long slots = 0;
+  for (uint32_t c0 = 0; c0 < mine; c0 += 64) {
+    const uint32_t nr = min(64u, mine - c0);
+    uint32_t j = 0, deg = 0;
+    uint64_t beg = 0;
+    if (lane < nr) {
+      j = s.spikes[b * kBlock + q + (c0 + lane) * P];
+      if (j < p.n_exc) {
+        beg = s.csc_ptr[j];
+        deg = (uint32_t)(s.csc_ptr[j + 1] - beg);
+      }
+    }
+    const uint32_t total = chunk_setup(rc, lane, deg, beg, j);
+    slots += total;
+    uint32_t r = 0;
+    for (uint32_t f = lane; f < total; f += 128) {
+      const bool two = f + 64 < total;
+      const uint64_t e0 = chunk_slot(rc, f, r);
+      uint32_t r1 = r;
+      const uint64_t e1 = two ? chunk_slot(rc, f + 64, r1) : e0;
+      const uint32_t i0 = s.csc_src[e0], i1 = s.csc_src[e1];
+      const uint32_t k0 = s.csc_slot[e0], k1 = s.csc_slot[e1];
+      const TraceRec a0 = s.pre[i0], a1 = s.pre[i1];
+      // a source that fired at t owns this slot in its propagation item
+      const bool own0 = a0.s != t, own1 = two && a1.s != t;
+      float w0 = 0.0f, w1 = 0.0f;
+      if (own0) w0 = s.w[k0];
+      if (own1) w1 = s.w[k1];
+      if (own0) {
+        const float x = trace_now(a0.v, a0.s, s.ptab, p.ptab_n, t);
+        const float dlt = p.a_minus * x;
+        const float wn = w0 - dlt;
+        s.w[k0] = clampf_(wn, p.w_min, p.w_max);
+      }
+      if (own1) {
+        const float x = trace_now(a1.v, a1.s, s.ptab, p.ptab_n, t);
+        const float dlt = p.a_minus * x;
+        const float wn = w1 - dlt;
+        s.w[k1] = clampf_(wn, p.w_min, p.w_max);
+      }
+      r = r1;
+    }
+    wave_sync();
+  }
+  if (lane == 0) {
+    if (P == 1)
+      s.part[kPartLtd * p.nblk + b] += slots;
+    else
+      atomicAdd(&s.part[kPartLtd * p.nblk + b], slots);
+  }
+}
+
+// Work items: propagation (nblk * P) and depression (nblk_e * P), interleaved so that both
+// kinds are resident on every CU.
+template <bool PAIR>
+__global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, int32_t off) {
   __shared__ RowChunk sh[kBlock / 64];
   const int32_t t = *s.tbase + off;
   const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const uint32_t P = p.prop_split;
+  const uint32_t nprop = p.nblk * P;
+  const uint32_t nltd = PAIR ? ((p.n_exc + kBlock - 1) / kBlock) * P : 0u;
+  const uint32_t both = 2 * min(nprop, nltd);
+  const uint32_t nwork = nprop + nltd;
   const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
-  const uint32_t nblk_e = (p.n_exc + kBlock - 1) / kBlock;  // blocks holding excitatory neurons
-  RowChunk& rc = sh[wid];
-  for (uint32_t b = wave; b < nblk_e; b += nwaves) {
-    const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
-    if (cnt == 0) continue;
-    unsigned long long slots = 0;
-    for (uint32_t c0 = 0; c0 < cnt; c0 += 64) {
-      const uint32_t nr = min(64u, cnt - c0);
-      uint32_t j = 0, deg = 0;
-      uint64_t beg = 0;
-      const bool mine = lane < nr;
-      if (mine) {
-        j = s.spikes[b * kBlock + c0 + lane];
-        if (j < p.n_exc) {
-          beg = s.csc_ptr[j];
-          deg = (uint32_t)(s.csc_ptr[j + 1] - beg);
-        }
-      }
-      uint32_t incl = deg;
-#pragma unroll
-      for (uint32_t o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-      }
-      const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
-      rc.incl[lane] = incl;
-      rc.beg[lane] = beg;
-      wave_sync();
-      slots += total;
-      uint32_t r = 0;
-      for (uint32_t f = lane; f < total; f += 64) {
-        while (rc.incl[r] <= f) ++r;
-        const uint32_t excl = r ? rc.incl[r - 1] : 0u;
-        const uint64_t e = rc.beg[r] + (f - excl);
-        const uint32_t i = s.csc_src[e];
-        const uint32_t k = s.csc_slot[e];
-        const float q = trace_at(s.ptab, p.ptab_n, t - s.pre_s[i]);
-        const float x = s.pre_v[i] * q;
-        const float dlt = p.a_minus * x;
-        const float wn = s.w[k] - dlt;
-        s.w[k] = clampf_(wn, p.w_min, p.w_max);
-      }
-      if (mine && j < p.n_exc) {
-        const int32_t ps = s.post_s[j];
-        const float q = trace_at(s.mtab, p.mtab_n, t - ps);
-        float x = s.post_v[j] * q;
-        x = x * q;
-        s.post_v[j] = x + 1.0f;
-        s.post_s[j] = t;
-      }
-      wave_sync();
+  for (uint32_t wk = wave; wk < nwork; wk += nwaves) {
+    bool prop;
+    uint32_t item;
+    if (wk < both) {
+      prop = (wk & 1u) == 0u;
+      item = wk >> 1;
+    } else {
+      prop = nprop > nltd;
+      item = wk - both / 2;
     }
-    if (lane == 0) s.part[kPartLtd * p.nblk + b] += slots;
+    const uint32_t b = item / P, q = item - b * P;
+    if (prop)
+      propagate_item<PAIR>(p, s, sh[wid], t, b, q, lane);
+    else if (PAIR)
+      depress_item(p, s, sh[wid], t, b, q, lane);
   }
 }
 
@@ -448,8 +558,13 @@ UpdateFn pick_update(int model, int drive, bool pair) {
   }
 }
 
-// one wavefront per neuron block
-uint32_t prop_grid(const nsx_ctx* ctx) { return (ctx->p.nblk + kBlock / 64 - 1) / (kBlock / 64); }
+// one wavefront per work item
+uint32_t syn_grid(const nsx_ctx* ctx) {
+  const bool pair = ctx->cfg.plasticity == NSX_PLAST_PAIR;
+  const uint64_t nblk_e = (ctx->cfg.n_exc + kBlock - 1) / kBlock;
+  const uint64_t waves = ((uint64_t)ctx->p.nblk + (pair ? nblk_e : 0)) * ctx->p.prop_split;
+  return (uint32_t)((waves + kBlock / 64 - 1) / (kBlock / 64));
+}
 
 // Enqueue the launches of one step with offset `off` from *tbase.
 hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
@@ -457,19 +572,15 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
   const bool pair = ctx->cfg.plasticity == NSX_PLAST_PAIR;
   UpdateFn upd = pick_update((int)ctx->cfg.model, (int)ctx->cfg.drive, pair);
   const uint32_t gu = (uint32_t)((ctx->n + kBlock - 1) / kBlock);
-  const uint32_t gp = prop_grid(ctx);
+  const uint32_t gs = syn_grid(ctx);
   if (evs) (void)hipEventRecord(evs[0], ctx->stream);
   hipLaunchKernelGGL(upd, dim3(gu), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
   if (evs) (void)hipEventRecord(evs[1], ctx->stream);
   if (pair)
-    hipLaunchKernelGGL(k_propagate<true>, dim3(gp), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+    hipLaunchKernelGGL(k_synapses<true>, dim3(gs), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
   else
-    hipLaunchKernelGGL(k_propagate<false>, dim3(gp), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+    hipLaunchKernelGGL(k_synapses<false>, dim3(gs), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
   if (evs) (void)hipEventRecord(evs[2], ctx->stream);
-  if (pair) {
-    hipLaunchKernelGGL(k_ltd, dim3(gp), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
-    if (evs) (void)hipEventRecord(evs[3], ctx->stream);
-  }
   return hipGetLastError();
 }
 

@@ -165,6 +165,10 @@ class SimulationEngine:
     def set_graph_steps(self, g):
         self._chk(self._lib.nsx_set_graph_steps(self._h, int(g)), "nsx_set_graph_steps")
 
+    def set_prop_split(self, waves_per_block):
+        self._chk(self._lib.nsx_set_prop_split(self._h, int(waves_per_block)),
+                  "nsx_set_prop_split")
+
     def set_profiling(self, on):
         self._chk(self._lib.nsx_set_profiling(self._h, 1 if on else 0), "nsx_set_profiling")
 
@@ -172,7 +176,9 @@ class SimulationEngine:
         ms, n, b = C.c_double(), C.c_uint64(), C.c_uint64()
         self._chk(self._lib.nsx_kernel_time(self._h, kind, C.byref(ms), C.byref(n), C.byref(b)),
                   "nsx_kernel_time")
-        return ms.value, n.value
+        # (total ms, launches, STDP slots touched in the last nsx_step call: kind 1 potentiated,
+        #  kind 2 depressed)
+        return ms.value, n.value, b.value
 
     def raster(self, clear=False):
         navail = C.c_uint64()

@@ -35,6 +35,17 @@ def neuron_bytes(cfg, n, spikes):
     return per_n * n + per_s * spikes
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE / WRITE_SIZE runs kept in
+    profiles/ (rocprofv3 --pmc; commands in that file), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_izh_stdp_1M_100M.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(spec, gpu_events_per_step, steps):
     """FP64 restatement of the reference's step (per-synapse STDP traces, serial O(M) synapse
     passes as engine.rs:397-537) timed on one host core over a bounded window of the same
@@ -73,7 +84,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10_000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--neurons", type=int, default=1_000_000)
-    ap.add_argument("--graph", type=int, default=0, help="steps per captured hipGraph (0: off)")
+    ap.add_argument("--graph", type=int, default=50, help="steps per captured hipGraph (0: off)")
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
     ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
     ap.add_argument("--cpu-steps", type=int, default=8)
@@ -91,13 +102,15 @@ def main():
     eng = ns.SimulationEngine.from_spec(spec)
     build_s = time.perf_counter() - t0
     log(f"bench: {spec.name} n={spec.n} m={spec.m} built on device in {build_s:.2f}s")
-    if args.graph:
-        eng.set_graph_steps(args.graph)
     if args.split:
         eng.set_prop_split(args.split)
+    # per-kernel hipEvents bracket direct launches on the library's stream; without them the
+    # timed region replays captured graphs of --graph steps
+    prof = not args.no_profile
+    if args.graph and not prof:
+        eng.set_graph_steps(args.graph)
     eng.step(args.warmup)
     log(f"bench: warmup {args.warmup} steps done")
-    prof = not args.no_profile and not args.graph
     eng.set_profiling(prof)
     t0 = time.perf_counter()
     st = eng.step(args.steps)
@@ -123,7 +136,8 @@ def main():
             "workload": "izhikevich_pair_stdp_1M_100M",
             "neurons": spec.n, "synapses": spec.m, "dt_ms": spec.cfg.dt_ms,
             "v_substeps": spec.cfg.v_substeps, "delays_ms": [1, 20], "sim_ms": sim_s * 1e3,
-            "accumulate": "int64 fixed point 2^-32", "graph_steps": args.graph,
+            "accumulate": "int64 fixed point 2^-32",
+            "launch": "direct, hipEvents per kernel" if prof else f"hipGraph x{args.graph}",
         },
         "x_realtime": sim_s / wall,
         "spikes": spikes,
@@ -162,9 +176,22 @@ def main():
         out["roofline"] = {
             "kernel": dom, "bound": "hbm", "achieved": kt[dom]["achieved_GBs"],
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kt[dom]["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom),
         }
         out["kernels"] = kt
+        if args.graph:
+            # the same network continued with graph replay and no per-kernel events
+            eng.set_profiling(False)
+            eng.set_graph_steps(args.graph)
+            g_steps = min(args.steps, 3000)
+            t0 = time.perf_counter()
+            gst = eng.step(g_steps)
+            g_wall = time.perf_counter() - t0
+            out["graph_replay"] = {
+                "steps": g_steps, "graph_steps": args.graph, "ms_per_step": g_wall * 1e3 / g_steps,
+                "syn_events_per_s": gst.events / g_wall,
+                "x_realtime": g_steps * spec.cfg.dt_ms / 1000.0 / g_wall,
+            }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, events / args.steps, args.cpu_steps)
     eng.close()

@@ -672,6 +672,8 @@ static nsx_status read_totals(nsx_ctx* ctx, uint64_t tot[kNumPart]) {
   return NSX_OK;
 }
 
+// Capture G steps (+ the tbase advance) into one graph.  Per-kernel timing needs events on the
+// stream (HIP does not time events recorded by graph nodes), so profiled steps launch directly.
 static nsx_status capture_graph(nsx_ctx* ctx) {
   destroy_graph(ctx);
   const int32_t G = (int32_t)ctx->graph_steps;
@@ -732,6 +734,7 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
       }
     }
   }
+  const uint64_t first_direct = done;
   for (; done < n_steps; ++done) {
     hipEvent_t* evs = prof ? &ctx->ev_pool[nev * done] : nullptr;
     hipError_t e = enqueue_step(ctx, (int32_t)(ctx->t - (uint64_t)ctx->tbase_host), evs);
@@ -751,7 +754,7 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   (void)hipEventDestroy(e_beg);
   (void)hipEventDestroy(e_end);
   if (prof) {
-    for (uint64_t k = 0; k < n_steps; ++k) {
+    for (uint64_t k = first_direct; k < n_steps; ++k) {
       hipEvent_t* evs = &ctx->ev_pool[nev * k];
       for (size_t q = 0; q + 1 < nev; ++q) {
         float ms = 0.0f;

@@ -138,7 +138,7 @@ struct nsx_ctx {
   bool profiling = false;
   double kt_ms[3] = {0, 0, 0};
   uint64_t kt_launches[3] = {0, 0, 0};
-  std::vector<hipEvent_t> ev_pool;
+  std::vector<hipEvent_t> ev_pool;  // direct launches: 3 events per step
   uint32_t last_call_steps = 0;
   std::vector<unsigned long long> part_host;
   uint64_t part_delta[4] = {0, 0, 0, 0};

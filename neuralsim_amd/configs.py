@@ -134,12 +134,16 @@ def izh_stdp_network(n=1_000_000, seed=7, steps=10_000, plasticity=True, record_
 
 
 def hh_network(n=200_000, seed=11, steps=100_000, record_raster=False, raster_capacity=0,
-               k_in=100, w_exc=(0.0, 0.004), w_inh=(-0.02, 0.0), dt_ms=0.01):
-    """HH conductance network: AMPA/GABA-A/NMDA, OU drive (6, 3, 5 ms), dt 0.01 ms."""
+               k_in=100, w_exc=(0.0, 0.004), w_inh=(-0.02, 0.0), dt_ms=0.01, hh_substeps=1):
+    """HH conductance network: AMPA/GABA-A/NMDA, OU drive (6, 3, 5 ms), dt 0.01 ms.
+
+    ``hh_substeps`` > 1 integrates the membrane with that many Euler sub-steps of
+    dt/hh_substeps per step, synaptic input and drive held over the step."""
     c = base_config(abi.MODEL_HH)
     c.n_neurons = n
     c.n_exc = (n * 4) // 5
     c.dt_ms = dt_ms
+    c.hh_substeps = hh_substeps
     c.hh_spike_mode = abi.HH_SPIKE_CROSSING
     c.hh_spike_v = 0.0
     c.tau_exc_ms, c.tau_inh_ms, c.tau_nmda_ms = 2.0, 10.0, 100.0

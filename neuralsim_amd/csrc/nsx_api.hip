@@ -148,8 +148,11 @@ static nsx_status validate_config(const nsx_config* c) {
     if (!(lam >= 0) || poisson_table(lam, thr, kMaxPoisson) >= kMaxPoisson)
       return set_error(nullptr, NSX_E_INVALID, "Poisson mean per step too large (>~8)");
   }
-  if (c->model == NSX_MODEL_HH && c->hh_substeps > 1)
-    return set_error(nullptr, NSX_E_INVALID, "hh_substeps > 1 not supported");
+  if ((c->model == NSX_MODEL_HH || c->model == NSX_MODEL_HH_SURROGATE) &&
+      (c->hh_substeps < 1 || c->hh_substeps > 64))
+    return set_error(nullptr, NSX_E_INVALID, "hh_substeps must be in [1, 64]");
+  if (c->hh_spike_mode == NSX_HH_SPIKE_REFERENCE_RESET && c->hh_substeps > 1)
+    return set_error(nullptr, NSX_E_INVALID, "the reference reset rule takes one sub-step per step");
   return NSX_OK;
 }
 
@@ -183,13 +186,11 @@ static void fill_params(nsx_ctx* ctx) {
   p.izh_h = (float)(dt / c.v_substeps);
   p.dtf = (float)dt;
   p.substeps = c.v_substeps;
-  p.g_na = c.hh_g_na;
-  p.g_k = c.hh_g_k;
-  p.g_l = c.hh_g_l;
-  p.e_na = c.hh_e_na;
-  p.e_k = c.hh_e_k;
-  p.e_l = c.hh_e_l;
-  p.hh_kv = (float)(dt / c.hh_c_m);
+  const uint32_t hs = c.hh_substeps ? c.hh_substeps : 1u;
+  p.hh = HHConst{c.hh_g_na, c.hh_g_k, c.hh_g_l, c.hh_e_na, c.hh_e_k, c.hh_e_l,
+                 (float)(dt / hs), (float)(dt / hs / c.hh_c_m),
+                 c.hh_spike_mode == NSX_HH_SPIKE_REFERENCE_RESET ? 1u : 0u};
+  p.hh_substeps = hs;
   p.hh_spike_v = c.hh_spike_v;
   p.e_exc = c.e_exc;
   p.e_inh = c.e_inh;

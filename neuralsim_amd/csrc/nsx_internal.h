@@ -31,6 +31,14 @@ struct __attribute__((aligned(16))) PostRec {
   int32_t pad;
 };
 
+// Hodgkin-Huxley constants of one Euler sub-step.
+struct HHConst {
+  float g_na, g_k, g_l, e_na, e_k, e_l;
+  float dts;         // sub-step length (ms)
+  float kv;          // dts / C_m
+  uint32_t literal;  // 1: the reference code's current signs (hodgkin_huxley.rs:78-84)
+};
+
 // Everything a step kernel reads that is constant over a run (passed by value).
 struct StepParams {
   uint32_t n, n_exc, D, ch;
@@ -48,7 +56,9 @@ struct StepParams {
   float izh_h, dtf;
   uint32_t substeps;
   // HH
-  float g_na, g_k, g_l, e_na, e_k, e_l, hh_kv, hh_spike_v;
+  HHConst hh;
+  uint32_t hh_substeps;  // Euler sub-steps per step, input current held
+  float hh_spike_v;
   float e_exc, e_inh, gmax_ampa, gmax_nmda, gmax_gaba, mgc;
   uint32_t hh_reset_mode, use_nmda;
   // drive

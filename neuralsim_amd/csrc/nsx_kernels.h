@@ -1,0 +1,97 @@
+// nsx_kernels.h -- device helpers shared by the step kernels (plain models and the surrogate).
+#pragma once
+#include "nsx_device.h"
+#include "nsx_internal.h"
+
+namespace nsx {
+
+__device__ inline float trace_at(const float* tab, int32_t tab_n, int32_t k) {
+  return (k >= 0 && k < tab_n) ? tab[k] : 0.0f;
+}
+
+// Trace value at step t of a record last bumped at step s (lazy decay through the table).
+__device__ inline float trace_now(float v, int32_t s, const float* tab, int32_t tab_n,
+                                  int32_t t) {
+  const float q = trace_at(tab, tab_n, t - s);
+  return v * q;
+}
+
+// StdpTrace::on_pre_spike / on_post_spike (plasticity.rs:144-154) on a lazily decayed record:
+// the per-step decay and the elapsed-time factor are both P[t - s].
+__device__ inline TraceRec trace_bump(const TraceRec rec, const float* tab, int32_t tab_n,
+                                      int32_t t) {
+  const float q = trace_at(tab, tab_n, t - rec.s);
+  float x = rec.v * q;
+  x = x * q;
+  return TraceRec{x + 1.0f, t};
+}
+
+__device__ inline PostRec post_bump(const PostRec rec, const float* tab, int32_t tab_n,
+                                    int32_t t) {
+  const float q = trace_at(tab, tab_n, t - rec.s);
+  float x = rec.v * q;
+  x = x * q;
+  return PostRec{x + 1.0f, t, rec.last, 0};
+}
+
+// Per-step drive value for neuron i (value added to channel 0 / the input current).
+template <int DRIVE>
+__device__ inline void drive_draw(const StepParams& p, uint32_t i, int32_t t, float& a0,
+                                  float& z) {
+  if (DRIVE == NSX_DRIVE_NONE) return;
+  U4 r = draw(p.seed, i, (uint32_t)t, kStreamDrive);
+  if (DRIVE == NSX_DRIVE_POISSON) {
+    uint32_t c = 0;
+    for (int k = 0; k < p.npthr; ++k) c += (r.x >= p.pthr[k]) ? 1u : 0u;
+    float x = (float)c * p.drive_weight;
+    a0 = a0 + x;
+  } else {
+    z = normal_(r.x, r.y);
+  }
+}
+
+// Block-local compaction of the spiking neurons of this 256-neuron block into its spike segment
+// (and the raster when recording).  Called by every thread of the block.
+__device__ inline void block_compact(const StepParams& p, const DevPtrs& s, bool spiked,
+                                     uint32_t i, int32_t t, uint32_t blk) {
+  // Block-local compaction into this block's spike segment: wave ballots, wave counts in LDS,
+  // no global atomics (one atomic per wave on a shared counter serialises at the memory-side
+  // atomic unit once tens of thousands of waves fire).
+  __shared__ uint32_t s_wcnt[kBlock / 64];
+  const unsigned long long mask = __ballot(spiked);
+  const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
+  const uint32_t nset = (uint32_t)__popcll(mask);
+  const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wcnt[wid] = nset;
+  __syncthreads();
+  uint32_t off_w = 0, tot = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kBlock / 64; ++k) {
+    const uint32_t c = s_wcnt[k];
+    off_w += k < wid ? c : 0u;
+    tot += c;
+  }
+  if (spiked) s.spikes[blk * kBlock + off_w + below] = i;
+  if (threadIdx.x == 0) {
+    s.bcnt[blk] = tot;
+    if (tot) s.part[kPartSpikes * p.nblk + blk] += tot;
+  }
+  if (p.raster_on && mask) {  // test/analysis path: append (step, id) pairs
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long rb = 0;
+    if ((int)lane == leader) {
+      rb = atomicAdd(&s.counters[kCntRaster], (unsigned long long)nset);
+      atomicAdd(&s.step_counts[(uint32_t)t % p.step_counts_cap], nset);
+    }
+    rb = __shfl(rb, leader);
+    if (spiked) {
+      unsigned long long pos = rb + below;
+      if (pos < p.raster_cap) {
+        s.raster_step[pos] = (uint32_t)t;
+        s.raster_id[pos] = i;
+      }
+    }
+  }
+}
+
+}  // namespace nsx

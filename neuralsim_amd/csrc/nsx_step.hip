@@ -19,67 +19,11 @@
 // Kernels read the step index as *tbase + off so that a captured hipGraph of G steps can be
 // replayed (k_advance moves tbase by G at the end of the graph).
 #include "nsx_device.h"
+#include "nsx_hh.h"
 #include "nsx_internal.h"
+#include "nsx_kernels.h"
 
 namespace nsx {
-
-__device__ inline float trace_at(const float* tab, int32_t tab_n, int32_t k) {
-  return (k >= 0 && k < tab_n) ? tab[k] : 0.0f;
-}
-
-// Trace value at step t of a record last bumped at step s (lazy decay through the table).
-__device__ inline float trace_now(float v, int32_t s, const float* tab, int32_t tab_n,
-                                  int32_t t) {
-  const float q = trace_at(tab, tab_n, t - s);
-  return v * q;
-}
-
-// StdpTrace::on_pre_spike / on_post_spike (plasticity.rs:144-154) on a lazily decayed record:
-// the per-step decay and the elapsed-time factor are both P[t - s].
-__device__ inline TraceRec trace_bump(const TraceRec rec, const float* tab, int32_t tab_n,
-                                      int32_t t) {
-  const float q = trace_at(tab, tab_n, t - rec.s);
-  float x = rec.v * q;
-  x = x * q;
-  return TraceRec{x + 1.0f, t};
-}
-
-__device__ inline PostRec post_bump(const PostRec rec, const float* tab, int32_t tab_n,
-                                    int32_t t) {
-  const float q = trace_at(tab, tab_n, t - rec.s);
-  float x = rec.v * q;
-  x = x * q;
-  return PostRec{x + 1.0f, t, rec.last, 0};
-}
-
-// Per-step drive value for neuron i (value added to channel 0 / the input current).
-template <int DRIVE>
-__device__ inline void drive_draw(const StepParams& p, uint32_t i, int32_t t, float& a0,
-                                  float& z) {
-  if (DRIVE == NSX_DRIVE_NONE) return;
-  U4 r = draw(p.seed, i, (uint32_t)t, kStreamDrive);
-  if (DRIVE == NSX_DRIVE_POISSON) {
-    uint32_t c = 0;
-    for (int k = 0; k < p.npthr; ++k) c += (r.x >= p.pthr[k]) ? 1u : 0u;
-    float x = (float)c * p.drive_weight;
-    a0 = a0 + x;
-  } else {
-    z = normal_(r.x, r.y);
-  }
-}
-
-// alpha = scale * x / (1 - exp(-x/10)); near x = 0 the limit k * (1 + x/20).
-__device__ inline float hh_ratio(float x, float scale, float k) {
-  if (fabsf(x) < 1e-3f) {
-    float y = x * 0.05f;
-    y = 1.0f + y;
-    return k * y;
-  }
-  float e = expf_(x * -0.1f);
-  float den = 1.0f - e;
-  float num = scale * x;
-  return num / den;
-}
 
 template <int MODEL, int DRIVE, bool PAIR>
 __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
@@ -203,44 +147,10 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
         float q = p.drive_sigma * z;
         iext = p.drive_mean + q;
       }
-      float I = iext + isyn;
-      float am = hh_ratio(v + 40.0f, 0.1f, 1.0f);
-      float bm = 4.0f * expf_((v + 65.0f) * (-1.0f / 18.0f));
-      float ah = 0.07f * expf_((v + 65.0f) * -0.05f);
-      float eh = expf_((v + 35.0f) * -0.1f);
-      eh = 1.0f + eh;
-      float bh = 1.0f / eh;
-      float an = hh_ratio(v + 55.0f, 0.01f, 0.1f);
-      float bn = 0.125f * expf_((v + 65.0f) * -0.0125f);
+      const float I = iext + isyn;  // held over the step's Euler sub-steps
       float m = s.gm[i], h = s.gh[i], nn = s.gn[i];
-      float t1, t2;
-      t1 = am * (1.0f - m); t2 = bm * m; t1 = t1 - t2; t1 = t1 * p.dtf; m = m + t1;
-      t1 = ah * (1.0f - h); t2 = bh * h; t1 = t1 - t2; t1 = t1 * p.dtf; h = h + t1;
-      t1 = an * (1.0f - nn); t2 = bn * nn; t1 = t1 - t2; t1 = t1 * p.dtf; nn = nn + t1;
-      float m3 = m * m;
-      m3 = m3 * m;
-      float ina = p.g_na * m3;
-      ina = ina * h;
-      ina = ina * (p.e_na - v);
-      float n2 = nn * nn;
-      float n4 = n2 * n2;
-      float ik = p.g_k * n4;
-      ik = ik * (p.e_k - v);
-      float il = p.g_l * (p.e_l - v);
-      // reference-reset mode keeps the reference code's current signs (hodgkin_huxley.rs:78-84);
-      // standard mode integrates the documented membrane equation (hodgkin_huxley.rs:6-9)
-      float acc;
-      if (p.hh_reset_mode) {
-        acc = I - ina;
-        acc = acc - ik;
-        acc = acc - il;
-      } else {
-        acc = I + ina;
-        acc = acc + ik;
-        acc = acc + il;
-      }
-      acc = acc * p.hh_kv;
-      float vn = v + acc;
+      float vn = v;
+      for (uint32_t k = 0; k < p.hh_substeps; ++k) hh_substep(vn, m, h, nn, I, p.hh);
       if (p.hh_reset_mode) {
         spiked = vn >= -55.0f;
         if (spiked) vn = -70.0f;
@@ -264,44 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       }
     }
   }
-  // Block-local compaction into this block's spike segment: wave ballots, wave counts in LDS,
-  // no global atomics (one atomic per wave on a shared counter serialises at the memory-side
-  // atomic unit once tens of thousands of waves fire).
-  __shared__ uint32_t s_wcnt[kBlock / 64];
-  const unsigned long long mask = __ballot(spiked);
-  const uint32_t lane = __lane_id(), wid = threadIdx.x >> 6;
-  const uint32_t nset = (uint32_t)__popcll(mask);
-  const uint32_t below = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-  if (lane == 0) s_wcnt[wid] = nset;
-  __syncthreads();
-  uint32_t off_w = 0, tot = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < kBlock / 64; ++k) {
-    const uint32_t c = s_wcnt[k];
-    off_w += k < wid ? c : 0u;
-    tot += c;
-  }
-  if (spiked) s.spikes[blockIdx.x * kBlock + off_w + below] = i;
-  if (threadIdx.x == 0) {
-    s.bcnt[blockIdx.x] = tot;
-    if (tot) s.part[kPartSpikes * p.nblk + blockIdx.x] += tot;
-  }
-  if (p.raster_on && mask) {  // test/analysis path: append (step, id) pairs
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned long long rb = 0;
-    if ((int)lane == leader) {
-      rb = atomicAdd(&s.counters[kCntRaster], (unsigned long long)nset);
-      atomicAdd(&s.step_counts[(uint32_t)t % p.step_counts_cap], nset);
-    }
-    rb = __shfl(rb, leader);
-    if (spiked) {
-      unsigned long long pos = rb + below;
-      if (pos < p.raster_cap) {
-        s.raster_step[pos] = (uint32_t)t;
-        s.raster_id[pos] = i;
-      }
-    }
-  }
+  block_compact(p, s, spiked, i, t, blockIdx.x);
 }
 
 // Order LDS traffic of one wavefront (all lanes write, then any lane reads).

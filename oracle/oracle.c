@@ -560,9 +560,12 @@ uint64_t ref_net_run(ref_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* 
         } else if (cfg->drive == NSX_DRIVE_GAUSSIAN) {
           iext = cfg->drive_mean + cfg->drive_sigma * z;
         }
-        ref_hh_integrate(&r->v[i], &r->gm[i], &r->gh[i], &r->gn[i], cfg->hh_g_na, cfg->hh_g_k,
-                         cfg->hh_g_l, cfg->hh_e_na, cfg->hh_e_k, cfg->hh_e_l, cfg->hh_c_m, dt,
-                         iext + isyn, cfg->hh_spike_mode != NSX_HH_SPIKE_REFERENCE_RESET);
+        const uint32_t sub = cfg->hh_substeps ? cfg->hh_substeps : 1u;
+        for (uint32_t k = 0; k < sub; ++k) /* input current held over the sub-steps */
+          ref_hh_integrate(&r->v[i], &r->gm[i], &r->gh[i], &r->gn[i], cfg->hh_g_na, cfg->hh_g_k,
+                           cfg->hh_g_l, cfg->hh_e_na, cfg->hh_e_k, cfg->hh_e_l, cfg->hh_c_m,
+                           dt / sub, iext + isyn,
+                           cfg->hh_spike_mode != NSX_HH_SPIKE_REFERENCE_RESET);
         if (cfg->hh_spike_mode == NSX_HH_SPIKE_REFERENCE_RESET) {
           if (r->v[i] >= -55.0) { r->v[i] = -70.0; spiked = 1; }
         } else {
@@ -647,6 +650,8 @@ struct mir_net {
   int npthr;
   /* precomputed float constants (host double -> float, as the device does) */
   float dec_e, dec_i, dec_n, lif_k, ou_k1, ou_k2, ou_k3, izh_h, dtf, hh_kv, mgc, scale_in;
+  float hh_dts;
+  uint32_t hh_sub;
   int32_t refr_steps;
   double scale_out;
 };
@@ -688,7 +693,9 @@ mir_net* mir_create(const nsx_config* cfg) {
   r->lif_k = (float)(dt / cfg->lif_tau_m);
   r->refr_steps = (int32_t)round((double)cfg->lif_t_ref_ms / dt);
   r->izh_h = (float)(dt / (cfg->v_substeps ? cfg->v_substeps : 1));
-  r->hh_kv = (float)(dt / cfg->hh_c_m);
+  r->hh_sub = cfg->hh_substeps ? cfg->hh_substeps : 1u;
+  r->hh_dts = (float)(dt / r->hh_sub);
+  r->hh_kv = (float)(dt / r->hh_sub / cfg->hh_c_m);
   r->mgc = (float)(cfg->mg_mM / 3.57);
   r->scale_in = ldexpf(1.0f, r->S);
   r->scale_out = ldexp(1.0, -r->S);
@@ -851,6 +858,40 @@ static inline float mir_ratio(float x, float scale_lin, float k) {
   return num / den;
 }
 
+/* One Euler sub-step with I held: gates at the old V, currents with the new gates and the old
+ * V (hodgkin_huxley.rs:46-86); literal = the code's current signs, else the documented
+ * equation. */
+static void mir_hh_substep(const nsx_config* cfg, float* pv, float* pm, float* ph, float* pn,
+                           float I, float dts, float kv, int literal) {
+  const float v = *pv;
+  float m = *pm, h = *ph, nn = *pn;
+  float am = mir_ratio(v + 40.0f, 0.1f, 1.0f);
+  float bm = 4.0f * orc_expf((v + 65.0f) * (-1.0f / 18.0f));
+  float ah = 0.07f * orc_expf((v + 65.0f) * -0.05f);
+  float eh = orc_expf((v + 35.0f) * -0.1f);
+  eh = 1.0f + eh;
+  float bh = 1.0f / eh;
+  float an = mir_ratio(v + 55.0f, 0.01f, 0.1f);
+  float bn = 0.125f * orc_expf((v + 65.0f) * -0.0125f);
+  float t1, t2;
+  t1 = am * (1.0f - m); t2 = bm * m; t1 = t1 - t2; t1 = t1 * dts; m = m + t1;
+  t1 = ah * (1.0f - h); t2 = bh * h; t1 = t1 - t2; t1 = t1 * dts; h = h + t1;
+  t1 = an * (1.0f - nn); t2 = bn * nn; t1 = t1 - t2; t1 = t1 * dts; nn = nn + t1;
+  float m3 = m * m; m3 = m3 * m;
+  float ina = cfg->hh_g_na * m3; ina = ina * h; ina = ina * (cfg->hh_e_na - v);
+  float n2 = nn * nn; float n4 = n2 * n2;
+  float ik = cfg->hh_g_k * n4; ik = ik * (cfg->hh_e_k - v);
+  float il = cfg->hh_g_l * (cfg->hh_e_l - v);
+  float acc;
+  if (literal) {
+    acc = I - ina; acc = acc - ik; acc = acc - il;
+  } else {
+    acc = I + ina; acc = acc + ik; acc = acc + il;
+  }
+  acc = acc * kv;
+  *pv = v + acc; *pm = m; *ph = h; *pn = nn;
+}
+
 static int mir_hh_update(const mir_net* r, uint64_t i, float a0, float a1, float z) {
   const nsx_config* cfg = &r->cfg;
   float v = r->v[i];
@@ -886,34 +927,12 @@ static int mir_hh_update(const mir_net* r, uint64_t i, float a0, float a1, float
     float q = cfg->drive_sigma * z;
     iext = cfg->drive_mean + q;
   }
-  float I = iext + isyn;
-  /* rates at the old V */
-  float am = mir_ratio(v + 40.0f, 0.1f, 1.0f);
-  float bm = 4.0f * orc_expf((v + 65.0f) * (-1.0f / 18.0f));
-  float ah = 0.07f * orc_expf((v + 65.0f) * -0.05f);
-  float eh = orc_expf((v + 35.0f) * -0.1f);
-  eh = 1.0f + eh;
-  float bh = 1.0f / eh;
-  float an = mir_ratio(v + 55.0f, 0.01f, 0.1f);
-  float bn = 0.125f * orc_expf((v + 65.0f) * -0.0125f);
+  float I = iext + isyn; /* held over the step's Euler sub-steps */
   float m = r->gm[i], h = r->gh[i], nn = r->gn[i];
-  float t1, t2;
-  t1 = am * (1.0f - m); t2 = bm * m; t1 = t1 - t2; t1 = t1 * r->dtf; m = m + t1;
-  t1 = ah * (1.0f - h); t2 = bh * h; t1 = t1 - t2; t1 = t1 * r->dtf; h = h + t1;
-  t1 = an * (1.0f - nn); t2 = bn * nn; t1 = t1 - t2; t1 = t1 * r->dtf; nn = nn + t1;
-  float m3 = m * m; m3 = m3 * m;
-  float ina = cfg->hh_g_na * m3; ina = ina * h; ina = ina * (cfg->hh_e_na - v);
-  float n2 = nn * nn; float n4 = n2 * n2;
-  float ik = cfg->hh_g_k * n4; ik = ik * (cfg->hh_e_k - v);
-  float il = cfg->hh_g_l * (cfg->hh_e_l - v);
-  float acc;
-  if (cfg->hh_spike_mode == NSX_HH_SPIKE_REFERENCE_RESET) {
-    acc = I - ina; acc = acc - ik; acc = acc - il;
-  } else {
-    acc = I + ina; acc = acc + ik; acc = acc + il;
-  }
-  acc = acc * r->hh_kv;
-  float vn = v + acc;
+  float vn = v;
+  const int literal = cfg->hh_spike_mode == NSX_HH_SPIKE_REFERENCE_RESET;
+  for (uint32_t k = 0; k < r->hh_sub; ++k)
+    mir_hh_substep(cfg, &vn, &m, &h, &nn, I, r->hh_dts, r->hh_kv, literal);
   int spiked;
   if (cfg->hh_spike_mode == NSX_HH_SPIKE_REFERENCE_RESET) {
     spiked = vn >= -55.0f;

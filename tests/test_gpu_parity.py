@@ -154,6 +154,16 @@ def test_hh_bitexact(nsx, orc):
     _assert_same(st, g, m)
 
 
+def test_hh_substeps_bitexact(nsx, orc):
+    """dt 0.05 ms with five held-input Euler sub-steps of 0.01 ms (the fused plain integrator)."""
+    spec = nsx.configs.hh_network(n=2000, seed=9, dt_ms=0.05, hh_substeps=5, record_raster=True,
+                                  raster_capacity=1 << 22)
+    fields = ["v", "m", "h", "n", "syn0", "syn1", "syn2", "drive"]
+    st, g, m = _run_pair(nsx, orc, spec, 800, fields)
+    assert st.spikes > 20
+    _assert_same(st, g, m)
+
+
 def test_hh_reference_reset_bitexact(nsx, orc):
     spec = nsx.configs.hh_network(n=1500, seed=6, record_raster=True, raster_capacity=1 << 22)
     spec.cfg.hh_spike_mode = nsx.abi.HH_SPIKE_REFERENCE_RESET

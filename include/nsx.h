@@ -262,6 +262,87 @@ nsx_status nsx_backend_step_neurons(nsx_ctx* ctx, uint32_t* spiked, float* membr
  * their CSR row (int64 fixed point, order-independent), read back input_current. */
 nsx_status nsx_backend_step_spmv(nsx_ctx* ctx, float* input_current);
 
+/* ---- learned HH surrogate -------------------------------------------------------------------
+ * No counterpart in the reference (which integrates HH only by forward Euler,
+ * src/neuron/hodgkin_huxley.rs:47-92).  A network step of the plain integrator is hh_substeps
+ * Euler sub-steps of dt_ms/hh_substeps with the input current held, each evaluating the six gate
+ * rates at its own V.  The surrogate step evaluates the rates once, at the step-start V, runs the
+ * same sub-steps with them frozen, and adds a learned correction: an MLP 5 -> 64 -> 64 -> 4 (ReLU)
+ * on the normalised (V, m, h, n, I) predicting (plain step - frozen-rate step) / (|dV| + eps) for
+ * (V, m, h, n), dV the frozen-rate step's change of V; gates are clamped to [0, 1].  Weights are bf16 (FP32 masters); arithmetic
+ * accumulates in FP32 (layers 2-3 on MFMA).  A context of model NSX_MODEL_HH_SURROGATE steps with
+ * it; neurons whose (V, m, h, n, I) leave the training envelope take the plain step instead.
+ * Generation and training need a context of model NSX_MODEL_HH or NSX_MODEL_HH_SURROGATE in
+ * crossing spike mode and use its HH constants, dt_ms, hh_substeps and OU drive.
+ *
+ * Parameter vector (NSX_SURROGATE_PARAMS floats): w1[64][5] b1[64] w2[64][64] b2[64] w3[4][64]
+ * b3[4].  Normalisation (NSX_SURROGATE_NORM floats): in_mu[5] in_isd[5] out_mu[4] out_sd[4]
+ * lo[5] hi[5] eps pad[3]; x_n = (x - in_mu) * in_isd, output y = y_n * out_sd + out_mu,
+ * correction = y * (|dV| + eps), envelope lo <= x <= hi per input. */
+#define NSX_SURROGATE_PARAMS 4804u
+#define NSX_SURROGATE_NORM 32u
+
+/* Training-set generator.  Trajectories of the plain integrator driven by OU currents: a
+ * fraction random_fraction with mean ~ U[mu_lo, mu_hi], sigma ~ U[0, sigma_hi],
+ * tau ~ U[tau_lo, tau_hi] (ms), the rest with the context's OU drive (6, 3, 5 ms without one);
+ * after warm_ms, every stride-th step's state is recorded with its target. */
+typedef struct nsx_gen_config {
+  uint32_t struct_size; /* sizeof(nsx_gen_config) */
+  uint32_t stride;
+  uint64_t n_samples;
+  uint64_t seed;
+  float random_fraction, scale_eps, warm_ms;
+  float mu_lo, mu_hi, sigma_hi, tau_lo, tau_hi;
+  uint32_t _reserved[8];
+} nsx_gen_config;
+
+typedef struct nsx_train_config {
+  uint32_t struct_size;     /* sizeof(nsx_train_config)                                     */
+  uint32_t steps;           /* Adam steps                                                   */
+  uint32_t waves;           /* wavefronts per step (one 64-sample chunk at a time)          */
+  uint32_t chunks_per_wave; /* minibatch = waves * chunks_per_wave * 64 samples             */
+  uint32_t init;            /* 1: re-initialise parameters (He-uniform) and Adam state      */
+  uint32_t sample_offset;   /* added to the step counter that keys minibatch sampling       */
+  float lr, lr_min;         /* cosine schedule from lr to lr_min over `steps`               */
+  float beta1, beta2, eps;  /* Adam                                                         */
+  uint32_t _pad;
+  uint64_t seed;            /* initialisation and minibatch sampling                        */
+  uint32_t _reserved[8];
+} nsx_train_config;
+
+/* Fill the context's training set with ~n_samples states visited by the plain integrator
+ * (OU-driven trajectories, half with the context's drive and half with random mean/sigma/tau)
+ * and their targets, and derive the normalisation and envelope from it.  *n_out
+ * receives the number stored. */
+nsx_status nsx_surrogate_generate(nsx_ctx* ctx, const nsx_gen_config* cfg, uint64_t* n_out);
+/* Train on the context's training set on the GPU (mean squared error of normalised increments).
+ * loss_history (optional) receives one float per step.  Deterministic for a given set and seed. */
+nsx_status nsx_surrogate_train(nsx_ctx* ctx, const nsx_train_config* cfg, float* loss_history);
+/* Read / load the FP32 parameters and the normalisation (either pointer may be NULL for get). */
+nsx_status nsx_surrogate_get(nsx_ctx* ctx, float* params, float* norm);
+nsx_status nsx_surrogate_set(nsx_ctx* ctx, const float* params, const float* norm);
+/* Evaluate the surrogate (the network step's code path) on count raw states x[count][5] =
+ * (V, m, h, n, I).  what = NSX_SURR_CORRECTION: y = the MLP's output in target units (the
+ * correction before the |dV| + eps factor);
+ * NSX_SURR_NEXT: y = next (V, m, h, n) as the network step computes it (plain step outside the
+ * envelope); NSX_SURR_FROZEN: y = next state of the frozen-rate sub-steps alone.
+ * inside[count] (optional) receives the envelope flags. */
+#define NSX_SURR_CORRECTION 0u
+#define NSX_SURR_NEXT 1u
+#define NSX_SURR_FROZEN 2u
+nsx_status nsx_surrogate_forward(nsx_ctx* ctx, const float* x, uint64_t count, uint32_t what,
+                                 float* y, uint8_t* inside);
+/* Gradient of the mean squared normalised error of the correction over (x[count][5],
+ * target[count][4]) computed by the training kernels (count a multiple of 64); grad in
+ * parameter-vector order. */
+nsx_status nsx_surrogate_gradient(nsx_ctx* ctx, const float* x, const float* target,
+                                  uint64_t count, float* grad, float* loss);
+/* Copy samples [first, first+count) of the training set: 9 floats each, the state (V m h n I)
+ * and the target (plain step - frozen-rate step of V m h n, divided by |dV| + eps). */
+nsx_status nsx_surrogate_read_data(nsx_ctx* ctx, uint64_t first, uint64_t count, float* out);
+/* Neuron-steps that took the plain fallback since the last nsx_reset_state. */
+nsx_status nsx_surrogate_fallbacks(nsx_ctx* ctx, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

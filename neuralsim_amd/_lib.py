@@ -66,6 +66,14 @@ PROTOTYPES = {
     "nsx_backend_set_model": (_int, [_VP, _u32]),
     "nsx_backend_step_neurons": (_int, [_VP, _P, _P, _P, _P, _P]),
     "nsx_backend_step_spmv": (_int, [_VP, _P]),
+    "nsx_surrogate_generate": (_int, [_VP, C.POINTER(abi.NsxGenConfig), C.POINTER(_u64)]),
+    "nsx_surrogate_train": (_int, [_VP, C.POINTER(abi.NsxTrainConfig), _P]),
+    "nsx_surrogate_get": (_int, [_VP, _P, _P]),
+    "nsx_surrogate_set": (_int, [_VP, _P, _P]),
+    "nsx_surrogate_forward": (_int, [_VP, _P, _u64, _u32, _P, _P]),
+    "nsx_surrogate_gradient": (_int, [_VP, _P, _P, _u64, _P, C.POINTER(_f32)]),
+    "nsx_surrogate_read_data": (_int, [_VP, _u64, _u64, _P]),
+    "nsx_surrogate_fallbacks": (_int, [_VP, C.POINTER(_u64)]),
 }
 
 

@@ -108,6 +108,49 @@ class NsxStepStats(C.Structure):
     ]
 
 
+SURROGATE_PARAMS = 4804
+SURROGATE_NORM = 32
+
+
+class NsxGenConfig(C.Structure):
+    """nsx_gen_config (include/nsx.h)."""
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("stride", C.c_uint32),
+        ("n_samples", C.c_uint64),
+        ("seed", C.c_uint64),
+        ("random_fraction", C.c_float),
+        ("scale_eps", C.c_float),
+        ("warm_ms", C.c_float),
+        ("mu_lo", C.c_float),
+        ("mu_hi", C.c_float),
+        ("sigma_hi", C.c_float),
+        ("tau_lo", C.c_float),
+        ("tau_hi", C.c_float),
+        ("_reserved", C.c_uint32 * 8),
+    ]
+
+
+class NsxTrainConfig(C.Structure):
+    """nsx_train_config (include/nsx.h)."""
+    _fields_ = [
+        ("struct_size", C.c_uint32),
+        ("steps", C.c_uint32),
+        ("waves", C.c_uint32),
+        ("chunks_per_wave", C.c_uint32),
+        ("init", C.c_uint32),
+        ("sample_offset", C.c_uint32),
+        ("lr", C.c_float),
+        ("lr_min", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("_pad", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("_reserved", C.c_uint32 * 8),
+    ]
+
+
 _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int32)
 _UP = C.POINTER(C.c_uint32)

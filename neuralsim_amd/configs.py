@@ -158,3 +158,26 @@ def hh_network(n=200_000, seed=11, steps=100_000, record_raster=False, raster_ca
     c.raster_capacity = raster_capacity
     return NetSpec(f"hh_{n}", c, k_in, w_exc, w_inh, (d_lo, d_hi), steps,
                    "HH + AMPA/GABA-A/NMDA(Mg 1 mM), OU 6/3/5 ms, delays 1-2 ms, dt 0.01 ms")
+
+
+def hh_surrogate_network(n=200_000, seed=11, steps=20_000, record_raster=False,
+                         raster_capacity=0, k_in=100, w_exc=(0.0, 0.004), w_inh=(-0.02, 0.0)):
+    """The HH network stepped by the learned surrogate: dt 0.05 ms, one MLP evaluation per step
+    standing for five held-input 0.01 ms Euler sub-steps (the plain step of
+    hh_network(dt_ms=0.05, hh_substeps=5), which is also its per-neuron fallback)."""
+    spec = hh_network(n=n, seed=seed, steps=steps, record_raster=record_raster,
+                      raster_capacity=raster_capacity, k_in=k_in, w_exc=w_exc, w_inh=w_inh,
+                      dt_ms=0.05, hh_substeps=5)
+    spec.cfg.model = abi.MODEL_HH_SURROGATE
+    spec.name = f"hh_surrogate_{n}"
+    spec.description = ("HH + AMPA/GABA-A/NMDA, OU 6/3/5 ms, dt 0.05 ms, learned surrogate "
+                        "5->64->64->4 for five 0.01 ms Euler sub-steps")
+    return spec
+
+
+# Training recipe of the surrogate: 50M transitions of the plain integrator (half under the
+# network's OU drive, half under random drives), targets scaled by 1 / (|dV| + 0.5 mV), Adam with
+# a cosine schedule over minibatches of 512 x 2 x 64 = 65,536 transitions.
+SURROGATE_RECIPE = {"n_samples": 50_000_000, "data_seed": 3, "random_fraction": 0.5,
+                    "scale_eps": 0.5, "steps": 20_000, "waves": 512, "chunks_per_wave": 2,
+                    "lr": 2e-3, "lr_min": 1e-5, "seed": 5}

@@ -123,8 +123,10 @@ static nsx_status validate_config(const nsx_config* c) {
   if (c->struct_size != sizeof(nsx_config))
     return set_error(nullptr, NSX_E_INVALID,
                      "nsx_config.struct_size mismatch (header/library version skew)");
-  if (c->model > NSX_MODEL_HH)
+  if (c->model > NSX_MODEL_HH_SURROGATE)
     return set_error(nullptr, NSX_E_INVALID, "unknown or unsupported model");
+  if (c->model == NSX_MODEL_HH_SURROGATE && c->hh_spike_mode != NSX_HH_SPIKE_CROSSING)
+    return set_error(nullptr, NSX_E_INVALID, "the HH surrogate needs the crossing spike mode");
   if (c->n_neurons < 1 || c->n_neurons >= 0xffffffffull)
     return set_error(nullptr, NSX_E_INVALID, "n_neurons must be in [1, 2^32-1)");
   if (c->n_exc > c->n_neurons) return set_error(nullptr, NSX_E_INVALID, "n_exc > n_neurons");
@@ -408,6 +410,9 @@ nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out) {
     d.ptab = pt;
     d.mtab = mt;
   }
+  if (cfg->model == NSX_MODEL_HH_SURROGATE &&
+      (s = dev_alloc(ctx, (void**)&ctx->sur.fb, (size_t)ctx->p.nblk * 8, true)))
+    goto fail;
   if ((s = nsx_reset_state(ctx))) goto fail;
   *out = ctx;
   return NSX_OK;
@@ -441,6 +446,8 @@ nsx_status nsx_reset_state(nsx_ctx* ctx) {
                               ctx->stream));
   NSX_HIP(ctx, hipMemsetAsync(ctx->d.counters, 0, kNumCounters * 8, ctx->stream));
   NSX_HIP(ctx, hipMemsetAsync(ctx->d.tbase, 0, 64, ctx->stream));
+  if (ctx->sur.fb)
+    NSX_HIP(ctx, hipMemsetAsync(ctx->sur.fb, 0, (size_t)ctx->p.nblk * 8, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->t = 0;
   ctx->tbase_host = 0;
@@ -695,6 +702,8 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (!ctx->has_network) return set_error(ctx, NSX_E_STATE, "no network loaded");
   if (ctx->t + n_steps >= (1ull << 30)) return set_error(ctx, NSX_E_INVALID, "step index limit");
+  if (ctx->cfg.model == NSX_MODEL_HH_SURROGATE && !(ctx->sur.has_params && ctx->sur.has_norm))
+    return set_error(ctx, NSX_E_STATE, "surrogate parameters not set (train or load them)");
   uint64_t c0[kNumPart];
   nsx_status s0 = read_totals(ctx, c0);
   if (s0) return s0;

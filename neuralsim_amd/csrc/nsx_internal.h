@@ -115,6 +115,24 @@ enum : int {
   kNumCounters = 8
 };
 
+// HH surrogate of a context (device buffers owned through nsx_ctx::allocs).
+struct Surrogate {
+  float* master = nullptr;  // FP32 parameters
+  float* adam_m = nullptr;
+  float* adam_v = nullptr;
+  void* image = nullptr;    // bf16 fragments + FP32 layer 1 / biases as the kernels read them
+  float* norm = nullptr;    // normalisation and envelope
+  float* loss = nullptr;
+  float* data = nullptr;    // training transitions, 9 floats each
+  uint64_t n_data = 0;
+  float* gpart = nullptr;   // per-wavefront partial gradients
+  float* lpart = nullptr;
+  uint32_t gpart_waves = 0;
+  unsigned long long* fb = nullptr;  // per-block fallback counts (surrogate model)
+  uint64_t adam_t = 0;
+  bool has_params = false, has_norm = false;
+};
+
 }  // namespace nsx
 
 struct nsx_ctx {
@@ -161,6 +179,7 @@ struct nsx_ctx {
   uint32_t* b_row_ptr32 = nullptr;
   float b_dt = 0.5f, b_time = 0.0f;
   uint32_t b_model = 0;
+  nsx::Surrogate sur;
 };
 
 // helpers shared by translation units
@@ -173,6 +192,7 @@ nsx_status build_csr_device(nsx_ctx* ctx, uint32_t* d_src, uint32_t* d_dst, floa
                             uint8_t* d_delay, uint64_t m);
 nsx_status build_csc_device(nsx_ctx* ctx);
 nsx_status validate_csr_device(nsx_ctx* ctx);
+hipError_t launch_update_surrogate(nsx_ctx* ctx, int32_t off);
 }  // namespace nsx
 
 #define NSX_HIP(ctx, expr)                                          \

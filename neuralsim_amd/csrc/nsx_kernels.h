@@ -50,6 +50,49 @@ __device__ inline void drive_draw(const StepParams& p, uint32_t i, int32_t t, fl
   }
 }
 
+// First thing of every update kernel with pair STDP: the post-synaptic trace bumps of this
+// block's spikes at step t-1 (its segment still holds them; it is rewritten at the end).
+template <bool PAIR>
+__device__ inline void update_prologue(const StepParams& p, const DevPtrs& s, int32_t t) {
+  if (!PAIR) return;
+  const uint32_t prev = s.bcnt[blockIdx.x];
+  if (threadIdx.x < prev) {
+    const uint32_t j = s.spikes[blockIdx.x * kBlock + threadIdx.x];
+    if (j < p.n_exc) s.post[j] = post_bump(s.post[j], s.mtab, p.mtab_n, t - 1);
+  }
+  __syncthreads();
+}
+
+// Read and clear neuron i's ring slot of step t: channel 0 into a0, channel 1 into a1 when the
+// network has two channels (TWO).
+template <bool TWO>
+__device__ inline void ring_take(const StepParams& p, const DevPtrs& s, uint32_t i, int32_t t,
+                                 float& a0, float& a1) {
+  const uint32_t slot = (uint32_t)t % p.D;
+  long long* r0 = s.ring + (size_t)slot * p.ch * p.n + i;
+  a0 = dequantize(*r0, p.scale_out);
+  *r0 = 0;
+  a1 = 0.0f;
+  if (TWO) {
+    long long* r1 = r0 + p.n;
+    a1 = dequantize(*r1, p.scale_out);
+    *r1 = 0;
+  }
+}
+
+// Bookkeeping of a spike of neuron i at step t: count, last spike step, pre-synaptic trace bump
+// and the post record's same-step flag (excitatory neurons, pair STDP).
+template <bool PAIR>
+__device__ inline void spike_record(const StepParams& p, const DevPtrs& s, uint32_t i,
+                                    int32_t t) {
+  s.count[i] += 1u;
+  s.last[i] = t;
+  if (PAIR && i < p.n_exc) {
+    s.pre[i] = trace_bump(s.pre[i], s.ptab, p.ptab_n, t);
+    s.post[i].last = t;
+  }
+}
+
 // Block-local compaction of the spiking neurons of this 256-neuron block into its spike segment
 // (and the raster when recording).  Called by every thread of the block.
 __device__ inline void block_compact(const StepParams& p, const DevPtrs& s, bool spiked,

@@ -1,0 +1,423 @@
+// nsx_mlp.h -- the HH surrogate MLP 5 -> 64 -> 64 -> 4 (ReLU) with bf16 weights and FP32
+// accumulation, evaluated and differentiated by one wavefront per chunk of 64 samples.
+//
+// Layer 1 (K = 5) runs on the vector ALUs in FP32 with the bf16-rounded weights: the normalised
+// inputs stay exact, and a K = 5 contraction would waste 11/16 of an MFMA.  Layers 2 and 3 run on
+// gfx950 bf16 MFMA (v_mfma_f32_32x32x16_bf16 / v_mfma_f32_16x16x32_bf16) with activations
+// rounded to bf16 and staged through LDS.  The output layer's weights are carried as two bf16
+// words, W3 = hi + lo (lo = bf16(W3 - hi)), and multiplied in two MFMA passes: its rounding
+// error lands directly on the outputs, and a single bf16 word there is the largest error of the
+// whole surrogate step.  Operand layouts:
+//   32x32x16: lane l holds A[m = l%32][k = 8(l/32)+j], B[k = 8(l/32)+j][n = l%32] (j < 8);
+//             C register i holds (m = 8(i/4) + 4(l/32) + i%4, n = l%32)
+//   16x16x32: lane l holds A[m = l%16][k = 8(l/16)+j], B[k = 8(l/16)+j][n = l%16];
+//             C register i holds (m = 4(l/16) + i, n = l%16)
+// Operand tiles are stored K-contiguous ([m][k] for A, [n][k] for B), so a lane's eight bf16 are
+// one 16-byte LDS read.  The weight operands are pre-arranged per lane in global memory (MlpImage)
+// so each wavefront loads them with coalesced 16-byte reads and no LDS.
+#pragma once
+#include <stddef.h>
+
+#include "nsx_device.h"
+
+namespace nsx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMlpIn = 5, kMlpH = 64, kMlpOut = 4;
+// FP32 master parameters: w1[64][5] b1[64] w2[64][64] b2[64] w3[4][64] b3[4]
+constexpr int kOffW1 = 0, kOffB1 = 320, kOffW2 = 384, kOffB2 = 4480, kOffW3 = 4544,
+              kOffB3 = 4800, kMlpParams = 4804;
+constexpr int kLdA = 72;  // bf16 row length of 64-wide tiles (rows 16-byte aligned, padded)
+constexpr int kLdO = 24;  // bf16 row length of 16-wide tiles
+
+// Normalisation and training envelope, raw units: x_n = (x - in_mu) * in_isd, output in target
+// units y = y_n * out_sd + out_mu; a state is inside the envelope when lo <= x <= hi.  The
+// surrogate applies y * (|dV| + scale_eps), dV the frozen-rate step's change of V.
+struct MlpNorm {
+  float in_mu[kMlpIn], in_isd[kMlpIn];
+  float out_mu[kMlpOut], out_sd[kMlpOut];
+  float lo[kMlpIn], hi[kMlpIn];
+  float scale_eps;
+  float pad[3];
+};
+constexpr int kNormFloats = (int)(sizeof(MlpNorm) / 4);
+
+// Weights as the kernels read them (rebuilt from the FP32 masters after every optimiser step).
+struct __attribute__((aligned(16))) MlpImage {
+  float w1[kMlpH * kMlpIn];  // bf16-rounded, as FP32, [h][k]
+  float b1[kMlpH], b2[kMlpH], b3[kMlpOut];
+  bf16x8 w2f[8][64];   // A fragments of W2 [h2][h1]: tile mt*4+ks (m-tile of 32, k-step of 16)
+  bf16x8 w3f[2][64];   // A fragments of W3 [o][h2] (hi words), o padded to 16: k-step of 32
+  bf16x8 w3lf[2][64];  // the same for the lo words
+  bf16x8 w2tf[8][64];  // A fragments of W2^T [h1][h2]: tile mt*4+ks
+  bf16x8 w3tf[2][64];  // A fragments of W3^T [h2][o] (hi words), o padded to 16: m-tile of 32
+  bf16x8 w3tlf[2][64]; // the same for the lo words
+};
+constexpr int kImageFrags = 24;  // fragment arrays of 64 lanes
+static_assert(offsetof(MlpImage, w2f) % 16 == 0, "fragment alignment");
+
+// The image's FP32 head and the normalisation, as staged into LDS by every block that evaluates
+// the MLP: read with wave-uniform addresses, they would otherwise be vector loads with a wait each
+// (the kernels' own global stores keep the compiler from proving them invariant for scalar loads).
+struct MlpScalars {
+  float w1[kMlpH * kMlpIn];
+  float b1[kMlpH], b2[kMlpH], b3[kMlpOut];
+  MlpNorm norm;
+};
+constexpr int kScalarHead = kMlpH * kMlpIn + 2 * kMlpH + kMlpOut;  // floats taken from the image
+constexpr int kScalarFloats = kScalarHead + kNormFloats;
+static_assert(offsetof(MlpImage, w2f) == kScalarHead * 4 && sizeof(MlpScalars) == kScalarFloats * 4,
+              "the image starts with w1, b1, b2, b3");
+
+// Cooperative copy by the whole block; the caller synchronises before use.
+__device__ inline void stage_scalars(MlpScalars& dst, const MlpImage* img, const MlpNorm* norm) {
+  float* d = reinterpret_cast<float*>(&dst);
+  const float* a = reinterpret_cast<const float*>(img);
+  const float* b = reinterpret_cast<const float*>(norm);
+  for (int k = threadIdx.x; k < kScalarFloats; k += blockDim.x)
+    d[k] = k < kScalarHead ? a[k] : b[k - kScalarHead];
+}
+
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline bf16x8 ld8(const __bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+// acc += A B^T over a 32x32x16 tile; A from a register fragment, B[n][k] from LDS (row ldb).
+__device__ inline void mma32(f32x16& acc, bf16x8 a, const __bf16* B, int ldb, uint32_t lane) {
+  const bf16x8 b = ld8(B + (lane & 31) * ldb + (lane >> 5) * 8);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+}
+// Same with A[m][k] from LDS (row lda).
+__device__ inline void mma32(f32x16& acc, const __bf16* A, int lda, const __bf16* B, int ldb,
+                             uint32_t lane) {
+  mma32(acc, ld8(A + (lane & 31) * lda + (lane >> 5) * 8), B, ldb, lane);
+}
+// acc += A B^T over a 16x16x32 tile.
+__device__ inline void mma16(f32x4& acc, bf16x8 a, const __bf16* B, int ldb, uint32_t lane) {
+  const bf16x8 b = ld8(B + (lane & 15) * ldb + (lane >> 4) * 8);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+}
+__device__ inline void mma16(f32x4& acc, const __bf16* A, int lda, const __bf16* B, int ldb,
+                             uint32_t lane) {
+  mma16(acc, ld8(A + (lane & 15) * lda + (lane >> 4) * 8), B, ldb, lane);
+}
+
+// Row (m) of C register i of a 32x32 tile.
+__device__ inline uint32_t c32_row(uint32_t i, uint32_t lane) {
+  return 8u * (i >> 2) + 4u * (lane >> 5) + (i & 3u);
+}
+
+// Layer 1 of the calling lane's sample in FP32: relu(b1 + W1 x) rounded to bf16, into a[0..63]
+// (a row of a [s][h] tile) and, when hs is given, into column s of an [h][s] tile.
+__device__ inline void layer1(const MlpScalars& W, const float x[kMlpIn], __bf16* a, __bf16* hs,
+                              uint32_t s) {
+#pragma unroll
+  for (int g = 0; g < 8; ++g) {
+    bf16x8 r;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int h = g * 8 + j;
+      float acc = W.b1[h];
+#pragma unroll
+      for (int k = 0; k < kMlpIn; ++k) acc = __builtin_fmaf(W.w1[h * kMlpIn + k], x[k], acc);
+      r[j] = (__bf16)(acc > 0.0f ? acc : 0.0f);
+      if (hs) hs[h * kLdA + s] = r[j];
+    }
+    *reinterpret_cast<bf16x8*>(a + g * 8) = r;
+  }
+}
+
+// Layer-2 pre-activations: acc[mt*2+nt] (h2 tile mt, sample tile nt) = W2 a1^T, a1 as [s][h1].
+__device__ inline void layer2(const MlpImage& W, const __bf16* a1_sh, f32x16 acc[4],
+                              uint32_t lane) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f32x16{};
+#pragma unroll
+  for (uint32_t ks = 0; ks < 4; ++ks)
+#pragma unroll
+    for (uint32_t mt = 0; mt < 2; ++mt) {
+      const bf16x8 a = W.w2f[mt * 4 + ks][lane];
+#pragma unroll
+      for (uint32_t nt = 0; nt < 2; ++nt)
+        mma32(acc[mt * 2 + nt], a, a1_sh + nt * 32 * kLdA + ks * 16, kLdA, lane);
+    }
+}
+
+// relu(acc + bias[m]) of a 32x32 tile (m = h0 + row, n = s0 + column) as bf16 into an [s][h]
+// tile (sh, 8-byte stores) and, when given, an [h][s] tile (hs).
+__device__ inline void store_relu(const f32x16& c, const float* bias, uint32_t h0, uint32_t s0,
+                                  __bf16* sh, __bf16* hs, uint32_t lane) {
+  const uint32_t s = s0 + (lane & 31);
+#pragma unroll
+  for (uint32_t g = 0; g < 4; ++g) {
+    bf16x4 r;
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      const uint32_t h = h0 + c32_row(4 * g + q, lane);
+      float x = c[4 * g + q] + bias[h];
+      r[q] = (__bf16)(x > 0.0f ? x : 0.0f);
+      if (hs) hs[h * kLdA + s] = r[q];
+    }
+    *reinterpret_cast<bf16x4*>(sh + s * kLdA + h0 + c32_row(4 * g, lane)) = r;
+  }
+}
+
+// LDS of one inference wavefront: [s][h] layer-1, then layer-2 activations; the 16 padding
+// bytes of row s (elements 64..71) receive sample s's four FP32 outputs.
+struct __attribute__((aligned(16))) InferLds {
+  __bf16 act[64 * kLdA];
+};
+__device__ inline f32x4* out_slot(InferLds& L, uint32_t s) {
+  return reinterpret_cast<f32x4*>(L.act + s * kLdA + 64);
+}
+
+// Normalised outputs of the calling lane's sample (normalised inputs xn).  Every lane of the
+// wavefront must call it (the MFMAs use all 64 lanes).
+__device__ inline void mlp_infer(const MlpImage& W, const MlpScalars& S, InferLds& L,
+                                 const float xn[kMlpIn], float y[kMlpOut], uint32_t lane) {
+  layer1(S, xn, L.act + lane * kLdA, nullptr, lane);
+  wave_lds_sync();
+  f32x16 acc[4];
+  layer2(W, L.act, acc, lane);
+  wave_lds_sync();  // every lane has read the layer-1 activations
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (uint32_t nt = 0; nt < 2; ++nt)
+      store_relu(acc[mt * 2 + nt], S.b2, mt * 32, nt * 32, L.act, nullptr, lane);
+  wave_lds_sync();
+  const bf16x8 w30 = W.w3f[0][lane], w31 = W.w3f[1][lane];
+  const bf16x8 w30l = W.w3lf[0][lane], w31l = W.w3lf[1][lane];
+#pragma unroll
+  for (uint32_t nt = 0; nt < 4; ++nt) {
+    f32x4 o = f32x4{};
+    mma16(o, w30l, L.act + nt * 16 * kLdA, kLdA, lane);
+    mma16(o, w31l, L.act + nt * 16 * kLdA + 32, kLdA, lane);
+    mma16(o, w30, L.act + nt * 16 * kLdA, kLdA, lane);
+    mma16(o, w31, L.act + nt * 16 * kLdA + 32, kLdA, lane);
+    if (lane < 16) {  // rows o = 0..3 of the output tile
+      f32x4 r;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) r[q] = o[q] + S.b3[q];
+      *out_slot(L, nt * 16 + lane) = r;  // padding bytes: no MFMA operand reads them
+    }
+  }
+  wave_lds_sync();
+  const f32x4 r = *out_slot(L, lane);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) y[q] = r[q];
+  wave_lds_sync();
+}
+
+// LDS of one training wavefront (forward and backward of a 64-sample chunk); 62,976 bytes.
+struct __attribute__((aligned(16))) TrainLds {
+  float x[64 * kMlpIn];  // normalised inputs [s][k]
+  float d3[64 * 4];      // normalised targets, then FP32 output gradients [s][o]
+  union {
+    struct {
+      __bf16 a1_sh[64 * kLdA];  // layer-1 activations [s][h1]
+      __bf16 a2_sh[64 * kLdA];  // layer-2 activations [s][h2]
+    } f;
+    float dz1[64 * 65];  // dL/dz1 [h1][s] (FP32, odd row length: conflict-free column walks)
+  } u;
+  __bf16 a1_hs[64 * kLdA];   // [h1][s]
+  __bf16 a2_hs[64 * kLdA];   // [h2][s]
+  __bf16 dz2_hs[64 * kLdA];  // dL/dz2 [h2][s]
+  __bf16 dz2_sh[64 * kLdA];  // dL/dz2 [s][h2]
+  __bf16 dy_os[16 * kLdA];   // dL/dy [o][s], rows 4..15 zero
+  __bf16 dy_so[64 * kLdO];   // dL/dy [s][o], columns 4..15 zero
+};
+static_assert(offsetof(TrainLds, u) % 16 == 0 && offsetof(TrainLds, a1_hs) % 16 == 0 &&
+                  offsetof(TrainLds, dy_os) % 16 == 0 && offsetof(TrainLds, dy_so) % 16 == 0,
+              "LDS tile alignment");
+
+// Gradient accumulators of one wavefront (MFMA C layouts for W2 and W3).
+struct GradAcc {
+  f32x16 w2[4];        // tile mt*2+nt: rows h2 (mt), columns h1 (nt)
+  f32x4 w3[4];         // tile nt: rows o (lanes 0..15), columns h2 = nt*16 + lane%16
+  float w1[kMlpIn];    // lane h1
+  float b1, b2, b3;    // lane h (b3: lanes 0..3)
+  float loss;          // sum of squared normalised errors of this lane's outputs
+};
+
+__device__ inline void grad_zero(GradAcc& G) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    G.w2[t] = f32x16{};
+    G.w3[t] = f32x4{};
+  }
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) G.w1[k] = 0.0f;
+  G.b1 = G.b2 = G.b3 = G.loss = 0.0f;
+}
+
+// Forward and backward of one chunk.  The caller has written L.x (normalised inputs) and L.d3
+// (normalised targets) for its lane's sample and passes the same inputs in xn.  Loss is
+// gscale * sum (y - target)^2, so dL/dy = 2 gscale (y - target).  Backward treats the bf16
+// rounding of activations as the identity.
+__device__ inline void mlp_train_chunk(const MlpImage& W, const MlpScalars& S, TrainLds& L,
+                                       GradAcc& G, const float xn[kMlpIn], float gscale,
+                                       uint32_t lane) {
+  // ---- forward ----
+  layer1(S, xn, L.u.f.a1_sh + lane * kLdA, L.a1_hs, lane);
+  wave_lds_sync();
+  f32x16 acc[4];
+  layer2(W, L.u.f.a1_sh, acc, lane);
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (uint32_t nt = 0; nt < 2; ++nt)
+      store_relu(acc[mt * 2 + nt], S.b2, mt * 32, nt * 32, L.u.f.a2_sh, L.a2_hs, lane);
+  wave_lds_sync();
+  {
+    const bf16x8 w30 = W.w3f[0][lane], w31 = W.w3f[1][lane];
+    const bf16x8 w30l = W.w3lf[0][lane], w31l = W.w3lf[1][lane];
+#pragma unroll
+    for (uint32_t nt = 0; nt < 4; ++nt) {
+      f32x4 o = f32x4{};
+      mma16(o, w30l, L.u.f.a2_sh + nt * 16 * kLdA, kLdA, lane);
+      mma16(o, w31l, L.u.f.a2_sh + nt * 16 * kLdA + 32, kLdA, lane);
+      mma16(o, w30, L.u.f.a2_sh + nt * 16 * kLdA, kLdA, lane);
+      mma16(o, w31, L.u.f.a2_sh + nt * 16 * kLdA + 32, kLdA, lane);
+      const uint32_t s = nt * 16 + (lane & 15);
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t row = 4 * (lane >> 4) + q;  // output index; rows >= 4 are padding
+        float d = 0.0f;
+        if (row < 4) {
+          const float e = (o[q] + S.b3[row]) - L.d3[s * 4 + row];
+          G.loss += e * e;
+          d = 2.0f * gscale * e;
+          L.d3[s * 4 + row] = d;
+        }
+        const __bf16 db = (__bf16)d;
+        L.dy_os[row * kLdA + s] = db;
+        L.dy_so[s * kLdO + row] = db;
+      }
+    }
+  }
+  wave_lds_sync();
+  // ---- backward ----
+  // dW3[o][h2] += dy[o][s] a2[h2][s]^T ; db3[o] += sum_s dy[o][s]
+#pragma unroll
+  for (uint32_t nt = 0; nt < 4; ++nt)
+#pragma unroll
+    for (uint32_t ks = 0; ks < 2; ++ks)
+      mma16(G.w3[nt], L.dy_os + ks * 32, kLdA, L.a2_hs + nt * 16 * kLdA + ks * 32, kLdA, lane);
+  if (lane < 4) {
+    float sb = 0.0f;
+    for (uint32_t s = 0; s < 64; ++s) sb += L.d3[s * 4 + lane];
+    G.b3 += sb;
+  }
+  // dz2[h2][s] = (W3^T dy)[h2][s] where a2 > 0
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt) {
+    const bf16x8 a = W.w3tf[mt][lane], al = W.w3tlf[mt][lane];
+#pragma unroll
+    for (uint32_t nt = 0; nt < 2; ++nt) {
+      acc[mt * 2 + nt] = f32x16{};
+      mma32(acc[mt * 2 + nt], al, L.dy_so + nt * 32 * kLdO, kLdO, lane);
+      mma32(acc[mt * 2 + nt], a, L.dy_so + nt * 32 * kLdO, kLdO, lane);
+    }
+  }
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (uint32_t nt = 0; nt < 2; ++nt) {
+      const uint32_t s = nt * 32 + (lane & 31);
+#pragma unroll
+      for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t h = mt * 32 + c32_row(i, lane);
+        const bool on = (float)L.a2_hs[h * kLdA + s] > 0.0f;
+        const __bf16 d = (__bf16)(on ? acc[mt * 2 + nt][i] : 0.0f);
+        L.dz2_hs[h * kLdA + s] = d;
+        L.dz2_sh[s * kLdA + h] = d;
+      }
+    }
+  wave_lds_sync();
+  {
+    float sb = 0.0f;
+    for (uint32_t s = 0; s < 64; ++s) sb += (float)L.dz2_hs[lane * kLdA + s];
+    G.b2 += sb;
+  }
+  // dW2[h2][h1] += dz2[h2][s] a1[h1][s]^T
+#pragma unroll
+  for (uint32_t ks = 0; ks < 4; ++ks)
+#pragma unroll
+    for (uint32_t mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (uint32_t nt = 0; nt < 2; ++nt)
+        mma32(G.w2[mt * 2 + nt], L.dz2_hs + mt * 32 * kLdA + ks * 16, kLdA,
+              L.a1_hs + nt * 32 * kLdA + ks * 16, kLdA, lane);
+  // dz1[h1][s] = (W2^T dz2)[h1][s] where a1 > 0 (FP32, into the dead a1_sh / a2_sh storage)
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f32x16{};
+#pragma unroll
+  for (uint32_t ks = 0; ks < 4; ++ks)
+#pragma unroll
+    for (uint32_t mt = 0; mt < 2; ++mt) {
+      const bf16x8 a = W.w2tf[mt * 4 + ks][lane];
+#pragma unroll
+      for (uint32_t nt = 0; nt < 2; ++nt)
+        mma32(acc[mt * 2 + nt], a, L.dz2_sh + nt * 32 * kLdA + ks * 16, kLdA, lane);
+    }
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (uint32_t nt = 0; nt < 2; ++nt) {
+      const uint32_t s = nt * 32 + (lane & 31);
+#pragma unroll
+      for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t h = mt * 32 + c32_row(i, lane);
+        const bool on = (float)L.a1_hs[h * kLdA + s] > 0.0f;
+        L.u.dz1[h * 65 + s] = on ? acc[mt * 2 + nt][i] : 0.0f;
+      }
+    }
+  wave_lds_sync();
+  // dW1[h1][k] += dz1[h1][s] x[s][k] ; db1[h1] += sum_s dz1[h1][s]   (lane h1, FP32)
+  {
+    float sb = 0.0f;
+    for (uint32_t s = 0; s < 64; ++s) {
+      const float d = L.u.dz1[lane * 65 + s];
+      sb += d;
+#pragma unroll
+      for (int k = 0; k < kMlpIn; ++k) G.w1[k] = __builtin_fmaf(d, L.x[s * kMlpIn + k], G.w1[k]);
+    }
+    G.b1 += sb;
+  }
+  wave_lds_sync();  // the next chunk overwrites every buffer
+}
+
+// Scatter a wavefront's accumulated gradient into master order (kMlpParams floats at out).
+__device__ inline void grad_store(const GradAcc& G, float* out, uint32_t lane) {
+#pragma unroll
+  for (int k = 0; k < kMlpIn; ++k) out[kOffW1 + lane * kMlpIn + k] = G.w1[k];
+  out[kOffB1 + lane] = G.b1;
+  out[kOffB2 + lane] = G.b2;
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (uint32_t nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t h2 = mt * 32 + c32_row(i, lane), h1 = nt * 32 + (lane & 31);
+        out[kOffW2 + h2 * 64 + h1] = G.w2[mt * 2 + nt][i];
+      }
+  if (lane < 16) {
+#pragma unroll
+    for (uint32_t nt = 0; nt < 4; ++nt)
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) out[kOffW3 + q * 64 + nt * 16 + lane] = G.w3[nt][q];
+  }
+  if (lane < 4) out[kOffB3 + lane] = G.b3;
+}
+
+}  // namespace nsx

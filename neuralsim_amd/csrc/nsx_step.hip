@@ -29,27 +29,11 @@ template <int MODEL, int DRIVE, bool PAIR>
 __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
   const int32_t t = *s.tbase + off;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (PAIR) {
-    // post trace bumps of this block's spikes at step t-1 (its segment still holds them)
-    const uint32_t prev = s.bcnt[blockIdx.x];
-    if (threadIdx.x < prev) {
-      const uint32_t j = s.spikes[blockIdx.x * kBlock + threadIdx.x];
-      if (j < p.n_exc) s.post[j] = post_bump(s.post[j], s.mtab, p.mtab_n, t - 1);
-    }
-    __syncthreads();  // the segment is rewritten below
-  }
+  update_prologue<PAIR>(p, s, t);
   bool spiked = false;
   if (i < p.n) {
-    const uint32_t slot = (uint32_t)t % p.D;
-    long long* r0 = s.ring + (size_t)slot * p.ch * p.n + i;
-    float a0 = dequantize(*r0, p.scale_out);
-    *r0 = 0;
-    float a1 = 0.0f;
-    if (MODEL != NSX_MODEL_IZHIKEVICH) {
-      long long* r1 = r0 + p.n;
-      a1 = dequantize(*r1, p.scale_out);
-      *r1 = 0;
-    }
+    float a0, a1;
+    ring_take<MODEL != NSX_MODEL_IZHIKEVICH>(p, s, i, t, a0, a1);
     float z = 0.0f;
     drive_draw<DRIVE>(p, i, t, a0, z);
     if (MODEL == NSX_MODEL_LIF) {
@@ -115,39 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       s.u[i] = u;
     } else {  // Hodgkin-Huxley with AMPA / GABA-A / NMDA conductances
       const float v = s.v[i];
-      float ga = s.s0[i] * p.dec_e;
-      float xa = a0 * p.gmax_ampa;
-      ga = ga + xa;
-      float gn = s.s2[i] * p.dec_n;
-      if (p.use_nmda) {
-        float e = expf_(-0.062f * v);
-        float bl = p.mgc * e;
-        bl = 1.0f + bl;
-        float xn = a0 * p.gmax_nmda;
-        xn = xn / bl;
-        gn = gn + xn;
-      }
-      float gg = s.s1[i] * p.dec_i;
-      float xg = fabsf(a1) * p.gmax_gaba;
-      gg = gg + xg;
-      float de = p.e_exc - v, di = p.e_inh - v;
-      float isyn = ga * de;
-      float tn = gn * de;
-      isyn = isyn + tn;
-      float tg = gg * di;
-      isyn = isyn + tg;
-      float iext = 0.0f;
-      if (DRIVE == NSX_DRIVE_OU) {
-        iext = s.drv[i];
-        float o = iext * p.ou_k1;
-        o = o + p.ou_k2;
-        float q = p.ou_k3 * z;
-        s.drv[i] = o + q;
-      } else if (DRIVE == NSX_DRIVE_GAUSSIAN) {
-        float q = p.drive_sigma * z;
-        iext = p.drive_mean + q;
-      }
-      const float I = iext + isyn;  // held over the step's Euler sub-steps
+      const float I = hh_input<DRIVE>(p, s, i, v, a0, a1, z);  // held over the sub-steps
       float m = s.gm[i], h = s.gh[i], nn = s.gn[i];
       float vn = v;
       for (uint32_t k = 0; k < p.hh_substeps; ++k) hh_substep(vn, m, h, nn, I, p.hh);
@@ -161,18 +113,8 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       s.gm[i] = m;
       s.gh[i] = h;
       s.gn[i] = nn;
-      s.s0[i] = ga;
-      s.s1[i] = gg;
-      s.s2[i] = gn;
     }
-    if (spiked) {
-      s.count[i] += 1u;
-      s.last[i] = t;
-      if (PAIR && i < p.n_exc) {
-        s.pre[i] = trace_bump(s.pre[i], s.ptab, p.ptab_n, t);
-        s.post[i].last = t;
-      }
-    }
+    if (spiked) spike_record<PAIR>(p, s, i, t);
   }
   block_compact(p, s, spiked, i, t, blockIdx.x);
 }
@@ -447,7 +389,12 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
   const uint32_t gu = (uint32_t)((ctx->n + kBlock - 1) / kBlock);
   const uint32_t gs = syn_grid(ctx);
   if (evs) (void)hipEventRecord(evs[0], ctx->stream);
-  hipLaunchKernelGGL(upd, dim3(gu), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+  if (ctx->cfg.model == NSX_MODEL_HH_SURROGATE) {
+    const hipError_t e = launch_update_surrogate(ctx, off);
+    if (e != hipSuccess) return e;
+  } else {
+    hipLaunchKernelGGL(upd, dim3(gu), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+  }
   if (evs) (void)hipEventRecord(evs[1], ctx->stream);
   if (pair)
     hipLaunchKernelGGL(k_synapses<true>, dim3(gs), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);

@@ -205,6 +205,91 @@ class SimulationEngine:
         return {"total_spikes": st.total_spikes, "total_events": st.total_events,
                 "sim_time_ms": st.sim_time_ms}
 
+    # ---- learned HH surrogate (no reference counterpart) ----------------------------------
+    def surrogate_generate(self, n_samples, seed=1, stride=7, random_fraction=0.5,
+                           scale_eps=0.5, warm_ms=20.0, mu=(-5.0, 30.0), sigma_hi=8.0,
+                           tau=(1.0, 10.0)):
+        """Training set from this context's plain HH step (nsx_gen_config); returns its size."""
+        g = abi.NsxGenConfig()
+        g.struct_size = C.sizeof(abi.NsxGenConfig)
+        g.n_samples, g.seed, g.stride = int(n_samples), int(seed), int(stride)
+        g.random_fraction, g.scale_eps, g.warm_ms = random_fraction, scale_eps, warm_ms
+        g.mu_lo, g.mu_hi = mu
+        g.sigma_hi = sigma_hi
+        g.tau_lo, g.tau_hi = tau
+        n = C.c_uint64()
+        self._chk(self._lib.nsx_surrogate_generate(self._h, C.byref(g), C.byref(n)),
+                  "nsx_surrogate_generate")
+        return n.value
+
+    def surrogate_train(self, steps, waves=512, chunks_per_wave=2, lr=2e-3, lr_min=1e-5,
+                        beta1=0.9, beta2=0.999, eps=1e-8, seed=1, init=True, sample_offset=0):
+        """Adam on the GPU; returns the per-step loss (mean squared normalised error)."""
+        tc = abi.NsxTrainConfig()
+        tc.struct_size = C.sizeof(abi.NsxTrainConfig)
+        tc.steps, tc.waves, tc.chunks_per_wave = int(steps), int(waves), int(chunks_per_wave)
+        tc.init, tc.sample_offset = int(bool(init)), int(sample_offset)
+        tc.lr, tc.lr_min, tc.beta1, tc.beta2, tc.eps = lr, lr_min, beta1, beta2, eps
+        tc.seed = int(seed)
+        hist = np.zeros(int(steps), np.float32)
+        self._chk(self._lib.nsx_surrogate_train(self._h, C.byref(tc), _ptr(hist)),
+                  "nsx_surrogate_train")
+        return hist
+
+    def surrogate_params(self):
+        """(FP32 parameter vector, normalisation vector)."""
+        p = np.empty(abi.SURROGATE_PARAMS, np.float32)
+        nm = np.empty(abi.SURROGATE_NORM, np.float32)
+        self._chk(self._lib.nsx_surrogate_get(self._h, _ptr(p), _ptr(nm)), "nsx_surrogate_get")
+        return p, nm
+
+    def surrogate_norm(self):
+        """Normalisation / envelope vector (available once training data exist)."""
+        nm = np.empty(abi.SURROGATE_NORM, np.float32)
+        self._chk(self._lib.nsx_surrogate_get(self._h, None, _ptr(nm)), "nsx_surrogate_get")
+        return nm
+
+    def set_surrogate_params(self, params, norm):
+        p = _c(params, np.float32)
+        nm = _c(norm, np.float32)
+        if p.size != abi.SURROGATE_PARAMS or nm.size != abi.SURROGATE_NORM:
+            raise ValueError("surrogate parameter / normalisation vector of the wrong size")
+        self._chk(self._lib.nsx_surrogate_set(self._h, _ptr(p), _ptr(nm)), "nsx_surrogate_set")
+
+    def surrogate_forward(self, x, what=0):
+        """Surrogate on raw states x (count x 5 = V m h n I).  what 0: the MLP's correction;
+        1: next (V, m, h, n) as the network step computes it; 2: next state of the frozen-rate
+        sub-steps alone.  Returns (count x 4, envelope flags)."""
+        x = _c(x, np.float32).reshape(-1, 5)
+        y = np.empty((x.shape[0], 4), np.float32)
+        inside = np.empty(x.shape[0], np.uint8)
+        self._chk(self._lib.nsx_surrogate_forward(self._h, _ptr(x), x.shape[0], int(what),
+                                                  _ptr(y), _ptr(inside)), "nsx_surrogate_forward")
+        return y, inside.astype(bool)
+
+    def surrogate_gradient(self, x, target):
+        """Gradient (parameter order) and loss of the mean squared normalised error."""
+        x = _c(x, np.float32).reshape(-1, 5)
+        t = _c(target, np.float32).reshape(-1, 4)
+        g = np.empty(abi.SURROGATE_PARAMS, np.float32)
+        loss = C.c_float()
+        self._chk(self._lib.nsx_surrogate_gradient(self._h, _ptr(x), _ptr(t), x.shape[0],
+                                                   _ptr(g), C.byref(loss)),
+                  "nsx_surrogate_gradient")
+        return g, loss.value
+
+    def surrogate_data(self, first, count):
+        out = np.empty((int(count), 9), np.float32)
+        self._chk(self._lib.nsx_surrogate_read_data(self._h, int(first), int(count), _ptr(out)),
+                  "nsx_surrogate_read_data")
+        return out
+
+    def surrogate_fallbacks(self):
+        n = C.c_uint64()
+        self._chk(self._lib.nsx_surrogate_fallbacks(self._h, C.byref(n)),
+                  "nsx_surrogate_fallbacks")
+        return n.value
+
 
 class GpuBackend:
     """Drop-in for the wgpu GpuBackend (gpu_backend.rs:47-95), same methods and semantics."""

@@ -861,22 +861,31 @@ static inline float mir_ratio(float x, float scale_lin, float k) {
 /* One Euler sub-step with I held: gates at the old V, currents with the new gates and the old
  * V (hodgkin_huxley.rs:46-86); literal = the code's current signs, else the documented
  * equation. */
-static void mir_hh_substep(const nsx_config* cfg, float* pv, float* pm, float* ph, float* pn,
-                           float I, float dts, float kv, int literal) {
-  const float v = *pv;
-  float m = *pm, h = *ph, nn = *pn;
-  float am = mir_ratio(v + 40.0f, 0.1f, 1.0f);
-  float bm = 4.0f * orc_expf((v + 65.0f) * (-1.0f / 18.0f));
-  float ah = 0.07f * orc_expf((v + 65.0f) * -0.05f);
+typedef struct {
+  float am, bm, ah, bh, an, bn;
+} mir_rates;
+
+static mir_rates mir_hh_rates(float v) {
+  mir_rates r;
+  r.am = mir_ratio(v + 40.0f, 0.1f, 1.0f);
+  r.bm = 4.0f * orc_expf((v + 65.0f) * (-1.0f / 18.0f));
+  r.ah = 0.07f * orc_expf((v + 65.0f) * -0.05f);
   float eh = orc_expf((v + 35.0f) * -0.1f);
   eh = 1.0f + eh;
-  float bh = 1.0f / eh;
-  float an = mir_ratio(v + 55.0f, 0.01f, 0.1f);
-  float bn = 0.125f * orc_expf((v + 65.0f) * -0.0125f);
+  r.bh = 1.0f / eh;
+  r.an = mir_ratio(v + 55.0f, 0.01f, 0.1f);
+  r.bn = 0.125f * orc_expf((v + 65.0f) * -0.0125f);
+  return r;
+}
+
+static void mir_hh_apply(const nsx_config* cfg, const mir_rates* r, float* pv, float* pm,
+                         float* ph, float* pn, float I, float dts, float kv, int literal) {
+  const float v = *pv;
+  float m = *pm, h = *ph, nn = *pn;
   float t1, t2;
-  t1 = am * (1.0f - m); t2 = bm * m; t1 = t1 - t2; t1 = t1 * dts; m = m + t1;
-  t1 = ah * (1.0f - h); t2 = bh * h; t1 = t1 - t2; t1 = t1 * dts; h = h + t1;
-  t1 = an * (1.0f - nn); t2 = bn * nn; t1 = t1 - t2; t1 = t1 * dts; nn = nn + t1;
+  t1 = r->am * (1.0f - m); t2 = r->bm * m; t1 = t1 - t2; t1 = t1 * dts; m = m + t1;
+  t1 = r->ah * (1.0f - h); t2 = r->bh * h; t1 = t1 - t2; t1 = t1 * dts; h = h + t1;
+  t1 = r->an * (1.0f - nn); t2 = r->bn * nn; t1 = t1 - t2; t1 = t1 * dts; nn = nn + t1;
   float m3 = m * m; m3 = m3 * m;
   float ina = cfg->hh_g_na * m3; ina = ina * h; ina = ina * (cfg->hh_e_na - v);
   float n2 = nn * nn; float n4 = n2 * n2;
@@ -890,6 +899,29 @@ static void mir_hh_substep(const nsx_config* cfg, float* pv, float* pm, float* p
   }
   acc = acc * kv;
   *pv = v + acc; *pm = m; *ph = h; *pn = nn;
+}
+
+static void mir_hh_substep(const nsx_config* cfg, float* pv, float* pm, float* ph, float* pn,
+                           float I, float dts, float kv, int literal) {
+  const mir_rates r = mir_hh_rates(*pv);
+  mir_hh_apply(cfg, &r, pv, pm, ph, pn, I, dts, kv, literal);
+}
+
+/* FP32 mirrors of one network step's membrane integration for a single neuron: `substeps` plain
+ * Euler sub-steps, or the surrogate's frozen-rate sub-steps (rates at the step-start V). */
+void orc_hh_plain(const nsx_config* cfg, float* v, float* m, float* h, float* n, float I,
+                  uint32_t substeps) {
+  const double dt = cfg->dt_ms;
+  const float dts = (float)(dt / substeps), kv = (float)(dt / substeps / cfg->hh_c_m);
+  for (uint32_t k = 0; k < substeps; ++k) mir_hh_substep(cfg, v, m, h, n, I, dts, kv, 0);
+}
+
+void orc_hh_frozen(const nsx_config* cfg, float* v, float* m, float* h, float* n, float I,
+                   uint32_t substeps) {
+  const double dt = cfg->dt_ms;
+  const float dts = (float)(dt / substeps), kv = (float)(dt / substeps / cfg->hh_c_m);
+  const mir_rates r = mir_hh_rates(*v);
+  for (uint32_t k = 0; k < substeps; ++k) mir_hh_apply(cfg, &r, v, m, h, n, I, dts, kv, 0);
 }
 
 static int mir_hh_update(const mir_net* r, uint64_t i, float a0, float a1, float z) {

@@ -27,6 +27,12 @@ float orc_cospi(float x);
 float orc_normal(uint32_t r0, uint32_t r1);       /* Box-Muller, float, own math        */
 double orc_normal64(uint32_t r0, uint32_t r1);    /* same uniforms, libm double math     */
 int orc_poisson_table(double lambda, uint32_t* thr, int cap); /* returns entries        */
+/* FP32 mirror of one step's HH membrane integration of a single neuron (crossing mode): plain
+ * Euler sub-steps, or the surrogate's sub-steps with rates frozen at the step-start V. */
+void orc_hh_plain(const nsx_config* cfg, float* v, float* m, float* h, float* n, float I,
+                  uint32_t substeps);
+void orc_hh_frozen(const nsx_config* cfg, float* v, float* m, float* h, float* n, float I,
+                   uint32_t substeps);
 
 /* ---- FP64 reference restatement ------------------------------------------------------ */
 int ref_lif_step(double* v, int32_t* refr, double resting, double threshold, double reset,

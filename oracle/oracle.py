@@ -26,6 +26,8 @@ PROTOS = {
     "orc_normal": (_f32, [_u32, _u32]),
     "orc_normal64": (_f64, [_u32, _u32]),
     "orc_poisson_table": (_int, [_f64, _P, _int]),
+    "orc_hh_plain": (None, [C.POINTER(abi.NsxConfig), _P, _P, _P, _P, _f32, _u32]),
+    "orc_hh_frozen": (None, [C.POINTER(abi.NsxConfig), _P, _P, _P, _P, _f32, _u32]),
     "ref_lif_step": (_int, [_dp, C.POINTER(_i32), _f64, _f64, _f64, _f64, _f64, _f64, _f64, _f64]),
     "ref_izh_step": (_int, [_dp, _dp, _f64, _f64, _f64, _f64, _f64, _f64]),
     "ref_hh_alpha_m": (_f64, [_f64]), "ref_hh_beta_m": (_f64, [_f64]),
@@ -119,6 +121,23 @@ def generate_list(spec):
     cfg = spec.cfg.copy()
     lib().mir_generate_list(C.byref(cfg), *spec.generator_args(), _p(src), _p(dst), _p(w), _p(d))
     return src, dst, w, d
+
+
+def hh_step(cfg, state, I, substeps, frozen=False):
+    """One step of the FP32 HH membrane integration per neuron (loop over a few neurons):
+    plain Euler sub-steps or the frozen-rate sub-steps.  state: (count, 4) V m h n."""
+    fn = lib().orc_hh_frozen if frozen else lib().orc_hh_plain
+    out = np.array(state, np.float32).reshape(-1, 4).copy()
+    I = np.asarray(I, np.float32).reshape(-1)
+    c = cfg.copy()
+    cells = [np.zeros(1, np.float32) for _ in range(4)]
+    for i in range(out.shape[0]):
+        for k in range(4):
+            cells[k][0] = out[i, k]
+        fn(C.byref(c), *[_p(x) for x in cells], float(I[i]), int(substeps))
+        for k in range(4):
+            out[i, k] = cells[k][0]
+    return out
 
 
 class MirrorNet:

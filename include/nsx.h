@@ -262,6 +262,48 @@ nsx_status nsx_backend_step_neurons(nsx_ctx* ctx, uint32_t* spiked, float* membr
  * their CSR row (int64 fixed point, order-independent), read back input_current. */
 nsx_status nsx_backend_step_spmv(nsx_ctx* ctx, float* input_current);
 
+/* ---- launch-shape autotuner --------------------------------------------------------------------
+ * No reference counterpart (the reference has one fixed wgpu dispatch shape, gpu_backend.rs:
+ * 921-981).  Times trial_steps steps for each candidate propagation split (wavefronts sharing a
+ * neuron block's spike segment: 1, 2, 4, 8, 16) at a 50-step graph window, then each graph window
+ * (10, 25, 100, 200 steps) at the best split, every trial from a device-side snapshot of the
+ * mutable state; restores the state and the step index, and sets the fastest pair. */
+typedef struct nsx_tune_report {
+  uint32_t prop_split, graph_steps; /* chosen                                                 */
+  uint32_t n_trials, _pad;
+  double best_us_per_step;
+  double us_per_step[32];           /* per trial, in trial order                              */
+  uint32_t trial_split[32], trial_graph[32];
+} nsx_tune_report;
+
+nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* report);
+
+/* ---- batched evolution population ----------------------------------------------------------
+ * Replaces the evaluation loop of Population::evolve_generation (src/evolution/population.rs:
+ * 116-124), which builds one CPU engine per genome (Genome::build_engine, genome.rs:151-169) and
+ * runs RateHomeostasis::evaluate (fitness.rs:59-75) on each in a rayon thread.  Here the
+ * networks are concatenated into one context (each padded to a multiple of 256 neurons) and
+ * stepped together by the ordinary nsx_step / hipGraph path.  Every random draw of network i is
+ * keyed by (seed_i, local neuron index, step), so its spikes, state and counts are bit-identical
+ * to a separate context created with the same config, n_neurons = n_i, seed = seed_i and
+ * nsx_generate_fixed_indegree(k_in, w ranges, d_lo, d_hi).  Raster ids, state and CSR of the
+ * batch use global indices: network i starts at the offset nsx_batch_layout reports. */
+typedef struct nsx_batch_net {
+  uint32_t n_neurons, n_exc, k_in, d_lo, d_hi, _pad;
+  uint64_t seed;
+  float w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi;
+} nsx_batch_net;
+
+/* base supplies the model, dt, drive and neuron parameters shared by all networks (its
+ * n_neurons, n_exc and seed are ignored); plasticity must be NSX_PLAST_NONE. */
+nsx_status nsx_batch_create(const nsx_config* base, const nsx_batch_net* nets, uint32_t n_nets,
+                            nsx_ctx** out);
+/* offsets[n_nets + 1]: first global neuron of each network; offsets[n_nets] = total neurons. */
+nsx_status nsx_batch_layout(nsx_ctx* ctx, uint64_t* offsets);
+/* Per network: total spikes since the last reset and the sum over its neurons of the squared
+ * per-neuron spike count (spikes_sq may be NULL).  Exact integers. */
+nsx_status nsx_batch_stats(nsx_ctx* ctx, uint64_t* spikes, uint64_t* spikes_sq);
+
 /* ---- learned HH surrogate -------------------------------------------------------------------
  * No counterpart in the reference (which integrates HH only by forward Euler,
  * src/neuron/hodgkin_huxley.rs:47-92).  A network step of the plain integrator is hh_substeps

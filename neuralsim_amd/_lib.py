@@ -66,6 +66,11 @@ PROTOTYPES = {
     "nsx_backend_set_model": (_int, [_VP, _u32]),
     "nsx_backend_step_neurons": (_int, [_VP, _P, _P, _P, _P, _P]),
     "nsx_backend_step_spmv": (_int, [_VP, _P]),
+    "nsx_autotune": (_int, [_VP, _u32, C.POINTER(abi.NsxTuneReport)]),
+    "nsx_batch_create": (_int, [C.POINTER(abi.NsxConfig), C.POINTER(abi.NsxBatchNet), _u32,
+                                C.POINTER(_VP)]),
+    "nsx_batch_layout": (_int, [_VP, _P]),
+    "nsx_batch_stats": (_int, [_VP, _P, _P]),
     "nsx_surrogate_generate": (_int, [_VP, C.POINTER(abi.NsxGenConfig), C.POINTER(_u64)]),
     "nsx_surrogate_train": (_int, [_VP, C.POINTER(abi.NsxTrainConfig), _P]),
     "nsx_surrogate_get": (_int, [_VP, _P, _P]),

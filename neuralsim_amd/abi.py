@@ -108,6 +108,37 @@ class NsxStepStats(C.Structure):
     ]
 
 
+class NsxTuneReport(C.Structure):
+    """nsx_tune_report (include/nsx.h)."""
+    _fields_ = [
+        ("prop_split", C.c_uint32),
+        ("graph_steps", C.c_uint32),
+        ("n_trials", C.c_uint32),
+        ("_pad", C.c_uint32),
+        ("best_us_per_step", C.c_double),
+        ("us_per_step", C.c_double * 32),
+        ("trial_split", C.c_uint32 * 32),
+        ("trial_graph", C.c_uint32 * 32),
+    ]
+
+
+class NsxBatchNet(C.Structure):
+    """nsx_batch_net (include/nsx.h)."""
+    _fields_ = [
+        ("n_neurons", C.c_uint32),
+        ("n_exc", C.c_uint32),
+        ("k_in", C.c_uint32),
+        ("d_lo", C.c_uint32),
+        ("d_hi", C.c_uint32),
+        ("_pad", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("w_exc_lo", C.c_float),
+        ("w_exc_hi", C.c_float),
+        ("w_inh_lo", C.c_float),
+        ("w_inh_hi", C.c_float),
+    ]
+
+
 SURROGATE_PARAMS = 4804
 SURROGATE_NORM = 32
 

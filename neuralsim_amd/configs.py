@@ -181,3 +181,60 @@ def hh_surrogate_network(n=200_000, seed=11, steps=20_000, record_raster=False,
 SURROGATE_RECIPE = {"n_samples": 50_000_000, "data_seed": 3, "random_fraction": 0.5,
                     "scale_eps": 0.5, "steps": 20_000, "waves": 512, "chunks_per_wave": 2,
                     "lr": 2e-3, "lr_min": 1e-5, "seed": 5}
+
+
+def _mix64(x):
+    """SplitMix64 finaliser (Steele, Lea & Flood 2014): per-network seeds from a master seed."""
+    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return x ^ (x >> 31)
+
+
+def population_base(record_raster=False, raster_capacity=0):
+    """Shared neuron/drive settings of the batched Izhikevich population: RS/FS 80/20 with the
+    c/d jitter of the 1M workload, dt 1 ms with two 0.5 ms V sub-steps, delays 1-10 ms (11 ring
+    slots), and a 5 Hz Poisson drive of 100 inputs x 10 (current units) per network (the
+    workload leaves the drive's size open)."""
+    c = izh_stdp_network(n=2, plasticity=False).cfg
+    c.ring_slots = 11
+    c.drive = abi.DRIVE_POISSON
+    c.drive_inputs = 100
+    c.drive_rate_hz = 5.0
+    c.drive_weight = 10.0
+    c.drive_mean = 0.0
+    c.drive_sigma = 0.0
+    c.record_raster = 1 if record_raster else 0
+    c.raster_capacity = raster_capacity
+    return c
+
+
+def population_nets(n_nets=256, seed=2026, n_range=(1000, 64000), k_range=(20, 200),
+                    w_exc=(0.0, 6.0), w_inh=(-5.0, 0.0), delays=(1, 10)):
+    """Networks of the batched population: N log-uniform in n_range, in-degree uniform in
+    k_range, 80/20 E/I, own seed each (SplitMix64 of the master seed and the index)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_nets)).astype(np.int64)
+    ks = rng.integers(k_range[0], k_range[1] + 1, n_nets)
+    nets = []
+    for i in range(n_nets):
+        b = abi.NsxBatchNet()
+        b.n_neurons = int(ns[i])
+        b.n_exc = int(ns[i]) * 4 // 5
+        b.k_in = int(ks[i])
+        b.d_lo, b.d_hi = delays
+        b.seed = _mix64((seed << 20) + i)
+        b.w_exc_lo, b.w_exc_hi = w_exc
+        b.w_inh_lo, b.w_inh_hi = w_inh
+        nets.append(b)
+    return nets
+
+
+def population_single(base, net):
+    """Config of one network of the population run on its own (the sequential reference)."""
+    c = base.copy()
+    c.n_neurons = net.n_neurons
+    c.n_exc = net.n_exc
+    c.seed = net.seed
+    return c

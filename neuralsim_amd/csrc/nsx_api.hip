@@ -12,6 +12,7 @@
 
 #include "nsx_device.h"
 #include "nsx_internal.h"
+#include "nsx_kernels.h"
 
 namespace nsx {
 
@@ -67,6 +68,7 @@ void dev_free(nsx_ctx* ctx, void* p) {
 __global__ void k_init_state(StepParams p, DevPtrs s, int model, nsx_config cfg) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= p.n) return;
+  const NeuronId id = neuron_id(p, s, i, i / kBlock);
   s.last[i] = kNeverStep;
   s.pre[i] = TraceRec{0.0f, kNeverStep};
   s.post[i] = PostRec{0.0f, kNeverStep, kNeverStep, 0};
@@ -78,10 +80,10 @@ __global__ void k_init_state(StepParams p, DevPtrs s, int model, nsx_config cfg)
   if (model == NSX_MODEL_LIF) {
     s.v[i] = p.v_rest;
   } else if (model == NSX_MODEL_IZHIKEVICH) {
-    U4 r = draw(p.seed, i, 0u, kStreamParam);
+    U4 r = draw(id.seed, id.local, 0u, kStreamParam);
     float x = u01(r.x);
     float x2 = x * x;
-    const int ty = i < p.n_exc ? 0 : 1;
+    const int ty = id.local < id.n_exc ? 0 : 1;
     float cj = cfg.izh_c_jitter * x2;
     float c = cfg.izh_c[ty] + cj;
     float dj = cfg.izh_d_jitter_frac * x2;

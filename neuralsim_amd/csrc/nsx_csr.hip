@@ -15,6 +15,7 @@
 
 #include "nsx_device.h"
 #include "nsx_internal.h"
+#include "nsx_kernels.h"
 
 namespace nsx {
 
@@ -61,18 +62,9 @@ __global__ void k_gen_fixed_indegree(uint64_t seed, uint32_t n, uint32_t n_exc, 
   uint64_t m = (uint64_t)n * k_in;
   if (idx >= m) return;
   uint32_t j = (uint32_t)(idx / k_in), k = (uint32_t)(idx % k_in);
-  U4 r = draw(seed, j, k, kStreamConn);
-  uint32_t s = (uint32_t)(((uint64_t)r.x * (uint64_t)(n - 1)) >> 32);
-  if (s >= j) s += 1;
-  float u = u01(r.y);
-  const bool exc = s < n_exc;
-  float lo = exc ? w_exc_lo : w_inh_lo, hi = exc ? w_exc_hi : w_inh_hi;
-  float span = hi - lo;
-  float x = span * u;
-  src[idx] = s;
+  gen_synapse(seed, n, n_exc, w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_span, j, k,
+              src[idx], w[idx], delay[idx]);
   dst[idx] = j;
-  w[idx] = lo + x;
-  delay[idx] = (uint8_t)(d_lo + (uint32_t)(((uint64_t)r.z * d_span) >> 32));
 }
 
 __global__ void k_check_csr(const uint64_t* row_ptr, const uint32_t* col, const uint8_t* delay,

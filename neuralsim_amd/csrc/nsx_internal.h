@@ -39,6 +39,17 @@ struct HHConst {
   uint32_t literal;  // 1: the reference code's current signs (hodgkin_huxley.rs:78-84)
 };
 
+// One network of a batched population (device table).  Networks are concatenated, each starting
+// at a multiple of kBlock, so every neuron block belongs to exactly one network.
+struct BatchNet {
+  uint64_t seed;       // the network's own RNG key (connectivity, parameters, drive)
+  uint64_t syn_off;    // first synapse of this network in the generated list
+  uint32_t offset;     // first global neuron
+  uint32_t n, n_exc;   // neurons, excitatory neurons (local indices [0, n_exc))
+  uint32_t k_in, d_lo, d_span;
+  float w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi;
+};
+
 // Everything a step kernel reads that is constant over a run (passed by value).
 struct StepParams {
   uint32_t n, n_exc, D, ch;
@@ -71,6 +82,8 @@ struct StepParams {
   uint32_t step_counts_cap;
   uint32_t raster_on;
   uint64_t raster_cap;
+  uint32_t batch;  // 1: neurons belong to the networks of a batched population
+  uint32_t n_nets;
 };
 
 // Device pointers used by the step kernels.
@@ -98,6 +111,16 @@ struct DevPtrs {
   const uint64_t* csc_ptr;
   const uint32_t* csc_slot;
   const uint32_t* csc_src;
+  // batched population
+  const uint32_t* blk_net;  // network of each neuron block
+  const BatchNet* bnet;
+};
+
+// Identity of a neuron for its RNG streams and E/I type: the network's seed and local index in
+// a batched population, the context's seed and the global index otherwise.
+struct NeuronId {
+  uint64_t seed;
+  uint32_t local, n, n_exc;
 };
 
 // planes of DevPtrs::part (each nblk entries; entry b written only by the wave owning block b)
@@ -180,6 +203,8 @@ struct nsx_ctx {
   float b_dt = 0.5f, b_time = 0.0f;
   uint32_t b_model = 0;
   nsx::Surrogate sur;
+  // batched population
+  std::vector<nsx::BatchNet> nets_h;
 };
 
 // helpers shared by translation units

@@ -34,12 +34,41 @@ __device__ inline PostRec post_bump(const PostRec rec, const float* tab, int32_t
   return PostRec{x + 1.0f, t, rec.last, 0};
 }
 
-// Per-step drive value for neuron i (value added to channel 0 / the input current).
+// Synapse k of target j of a fixed in-degree network (n neurons, [0, n_exc) excitatory): source
+// uniform over the other n-1 neurons, weight uniform in the source type's range, delay uniform in
+// [d_lo, d_lo + d_span).  Counter-based: the same (seed, j, k) gives the same synapse wherever it
+// is generated (alone or inside a batched population).
+__device__ inline void gen_synapse(uint64_t seed, uint32_t n, uint32_t n_exc, float w_exc_lo,
+                                   float w_exc_hi, float w_inh_lo, float w_inh_hi, uint32_t d_lo,
+                                   uint32_t d_span, uint32_t j, uint32_t k, uint32_t& src,
+                                   float& w, uint8_t& delay) {
+  U4 r = draw(seed, j, k, kStreamConn);
+  uint32_t s = (uint32_t)(((uint64_t)r.x * (uint64_t)(n - 1)) >> 32);
+  if (s >= j) s += 1;
+  float u = u01(r.y);
+  const bool exc = s < n_exc;
+  float lo = exc ? w_exc_lo : w_inh_lo, hi = exc ? w_exc_hi : w_inh_hi;
+  float span = hi - lo;
+  float x = span * u;
+  src = s;
+  w = lo + x;
+  delay = (uint8_t)(d_lo + (uint32_t)(((uint64_t)r.z * d_span) >> 32));
+}
+
+// RNG key, local index and network size of neuron i of block blk (see NeuronId).
+__device__ inline NeuronId neuron_id(const StepParams& p, const DevPtrs& s, uint32_t i,
+                                     uint32_t blk) {
+  if (!p.batch) return NeuronId{p.seed, i, p.n, p.n_exc};
+  const BatchNet& b = s.bnet[s.blk_net[blk]];
+  return NeuronId{b.seed, i - b.offset, b.n, b.n_exc};
+}
+
+// Per-step drive value of a neuron (value added to channel 0 / the input current).
 template <int DRIVE>
-__device__ inline void drive_draw(const StepParams& p, uint32_t i, int32_t t, float& a0,
+__device__ inline void drive_draw(const StepParams& p, const NeuronId& id, int32_t t, float& a0,
                                   float& z) {
   if (DRIVE == NSX_DRIVE_NONE) return;
-  U4 r = draw(p.seed, i, (uint32_t)t, kStreamDrive);
+  U4 r = draw(id.seed, id.local, (uint32_t)t, kStreamDrive);
   if (DRIVE == NSX_DRIVE_POISSON) {
     uint32_t c = 0;
     for (int k = 0; k < p.npthr; ++k) c += (r.x >= p.pthr[k]) ? 1u : 0u;

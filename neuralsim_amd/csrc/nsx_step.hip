@@ -30,12 +30,13 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
   const int32_t t = *s.tbase + off;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   update_prologue<PAIR>(p, s, t);
+  const NeuronId id = neuron_id(p, s, i, blockIdx.x);
   bool spiked = false;
-  if (i < p.n) {
+  if (i < p.n && id.local < id.n) {  // batch padding neurons stay idle
     float a0, a1;
     ring_take<MODEL != NSX_MODEL_IZHIKEVICH>(p, s, i, t, a0, a1);
     float z = 0.0f;
-    drive_draw<DRIVE>(p, i, t, a0, z);
+    drive_draw<DRIVE>(p, id, t, a0, z);
     if (MODEL == NSX_MODEL_LIF) {
       float ie = s.s0[i] * p.dec_e;
       ie = ie + a0;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
         u = u + s.d[i];
         spiked = true;
       } else {
-        const int ty = i < p.n_exc ? 0 : 1;
+        const int ty = id.local < id.n_exc ? 0 : 1;
         const float a = p.izh_a[ty], b = p.izh_b[ty];
         const float v0 = v;
         for (uint32_t k = 0; k < p.substeps; ++k) {

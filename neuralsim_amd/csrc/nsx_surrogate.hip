@@ -263,19 +263,30 @@ struct AdamArgs {
   float* loss_out;   // one float (the step's loss)
 };
 
-// 64 parameters per block; 4 rows of threads sum interleaved quarters of the waves, combined in
-// a fixed order.
-__global__ __launch_bounds__(256) void k_surr_reduce(AdamArgs a) {
-  __shared__ float part[4][64];
+// 64 parameters per block; 16 rows of threads each sum the waves w = row (mod 16) with four
+// independent accumulators (loads in flight), combined in a fixed order: deterministic.
+__global__ __launch_bounds__(1024) void k_surr_reduce(AdamArgs a) {
+  __shared__ float part[16][64];
   const uint32_t tx = threadIdx.x & 63u, ty = threadIdx.x >> 6;
   const uint32_t p = blockIdx.x * 64 + tx;
-  float g = 0.0f;
-  if (p < (uint32_t)kMlpParams)
-    for (uint32_t w = ty; w < a.waves; w += 4) g += a.gpart[(size_t)w * kMlpParams + p];
-  part[ty][tx] = g;
+  float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+  if (p < (uint32_t)kMlpParams) {
+    const float* src = a.gpart + p;
+    uint32_t w = ty;
+    for (; w + 48 < a.waves; w += 64) {
+      g0 += src[(size_t)w * kMlpParams];
+      g1 += src[(size_t)(w + 16) * kMlpParams];
+      g2 += src[(size_t)(w + 32) * kMlpParams];
+      g3 += src[(size_t)(w + 48) * kMlpParams];
+    }
+    for (; w < a.waves; w += 16) g0 += src[(size_t)w * kMlpParams];
+  }
+  part[ty][tx] = (g0 + g1) + (g2 + g3);
   __syncthreads();
   if (ty == 0 && p < (uint32_t)kMlpParams) {
-    g = ((part[0][tx] + part[1][tx]) + part[2][tx]) + part[3][tx];
+    float g = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) g += part[r][tx];
     if (a.adam) {
       const float m = a.beta1 * a.am[p] + (1.0f - a.beta1) * g;
       const float v = a.beta2 * a.av[p] + (1.0f - a.beta2) * g * g;
@@ -390,7 +401,8 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
   update_prologue<PAIR>(p, s, t);
   __syncthreads();
   const MlpNorm& nm = S.norm;
-  const bool live = i < p.n;
+  const NeuronId id = neuron_id(p, s, i, blockIdx.x);
+  const bool live = i < p.n && id.local < id.n;
   float x[kMlpIn];
 #pragma unroll
   for (int k = 0; k < kMlpIn; ++k) x[k] = nm.in_mu[k];
@@ -398,7 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
     float a0, a1;
     ring_take<true>(p, s, i, t, a0, a1);
     float z = 0.0f;
-    drive_draw<DRIVE>(p, i, t, a0, z);
+    drive_draw<DRIVE>(p, id, t, a0, z);
     x[0] = s.v[i];
     x[4] = hh_input<DRIVE>(p, s, i, x[0], a0, a1, z);
     x[1] = s.gm[i];
@@ -654,7 +666,7 @@ nsx_status nsx_surrogate_train(nsx_ctx* ctx, const nsx_train_config* tc, float* 
     aa.loss_out = hist + k;
     ta.step = (uint32_t)(at - 1) + tc->sample_offset;
     hipLaunchKernelGGL(k_surr_train, dim3(tc->waves), dim3(64), 0, ctx->stream, ta);
-    hipLaunchKernelGGL(k_surr_reduce, dim3((kMlpParams + 63) / 64), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_surr_reduce, dim3((kMlpParams + 63) / 64), dim3(1024), 0, ctx->stream,
                        aa);
     e = hipGetLastError();
     if (e == hipSuccess) e = sur_pack(ctx);
@@ -790,7 +802,7 @@ nsx_status nsx_surrogate_gradient(nsx_ctx* ctx, const float* x, const float* tar
   aa.loss_out = dg + kMlpParams;
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_surr_train, dim3(waves), dim3(64), 0, ctx->stream, ta);
-    hipLaunchKernelGGL(k_surr_reduce, dim3((kMlpParams + 63) / 64), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_surr_reduce, dim3((kMlpParams + 63) / 64), dim3(1024), 0, ctx->stream,
                        aa);
     e = hipGetLastError();
   }

@@ -169,6 +169,14 @@ class SimulationEngine:
         self._chk(self._lib.nsx_set_prop_split(self._h, int(waves_per_block)),
                   "nsx_set_prop_split")
 
+    def autotune(self, trial_steps=100):
+        """Measure and set the fastest propagation split and graph window (state unchanged);
+        returns (chosen split, chosen window, [(split, window, us per step), ...])."""
+        r = abi.NsxTuneReport()
+        self._chk(self._lib.nsx_autotune(self._h, int(trial_steps), C.byref(r)), "nsx_autotune")
+        trials = [(r.trial_split[k], r.trial_graph[k], r.us_per_step[k]) for k in range(r.n_trials)]
+        return r.prop_split, r.graph_steps, trials
+
     def set_profiling(self, on):
         self._chk(self._lib.nsx_set_profiling(self._h, 1 if on else 0), "nsx_set_profiling")
 
@@ -289,6 +297,41 @@ class SimulationEngine:
         self._chk(self._lib.nsx_surrogate_fallbacks(self._h, C.byref(n)),
                   "nsx_surrogate_fallbacks")
         return n.value
+
+
+class BatchedPopulation(SimulationEngine):
+    """Many independent networks stepped as one (nsx_batch_*): the evaluation side of
+    Population::evolve_generation (population.rs:116-124).  `nets` is a sequence of
+    NsxBatchNet; `base` supplies model, dt, drive and neuron parameters."""
+
+    def __init__(self, base, nets):
+        self._lib = load()
+        self.cfg = base.copy()
+        self.cfg.struct_size = C.sizeof(abi.NsxConfig)
+        arr = (abi.NsxBatchNet * len(nets))(*nets)
+        h = C.c_void_p()
+        check(self._lib.nsx_batch_create(C.byref(self.cfg), arr, len(nets), C.byref(h)), None,
+              "nsx_batch_create")
+        self._h = h
+        self.dt = float(base.dt_ms)
+        self.nets = list(nets)
+        self.offsets = np.empty(len(nets) + 1, np.uint64)
+        self._chk(self._lib.nsx_batch_layout(self._h, _ptr(self.offsets)), "nsx_batch_layout")
+        self.cfg.n_neurons = int(self.offsets[-1])
+        self.cfg.n_exc = int(self.offsets[-1])
+
+    def net_stats(self):
+        """Per network: (total spikes, sum of squared per-neuron spike counts)."""
+        k = len(self.nets)
+        sp, sq = np.empty(k, np.uint64), np.empty(k, np.uint64)
+        self._chk(self._lib.nsx_batch_stats(self._h, _ptr(sp), _ptr(sq)), "nsx_batch_stats")
+        return sp, sq
+
+    def split(self, global_ids):
+        """(network index, local index) of global neuron ids."""
+        g = np.asarray(global_ids, np.uint64)
+        net = np.searchsorted(self.offsets, g, side="right") - 1
+        return net, (g - self.offsets[net]).astype(np.uint32)
 
 
 class GpuBackend:

@@ -78,6 +78,100 @@ def cpu_baseline(spec, gpu_events_per_step, steps):
     }
 
 
+def _timed(eng, steps, graph):
+    eng.set_graph_steps(graph)
+    eng.step(max(graph, 1) * 2)  # capture and warm up
+    t0 = time.perf_counter()
+    st = eng.step(steps)
+    return time.perf_counter() - t0, st
+
+
+def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50):
+    """HH network of 200k neurons: the learned surrogate (dt 0.05 ms) against the plain
+    integrator at dt 0.01 ms and against plain five-sub-step fusion at dt 0.05 ms; per-kernel
+    MFMA rate of the surrogate update; spike displacement of the surrogate against the plain
+    FP32 integrator on an uncoupled population under the same drive."""
+    from neuralsim_amd import configs, surrogate
+    from neuralsim_amd.metrics import spike_displacement
+    res = {}
+    spec = configs.hh_surrogate_network(n=n)
+    with ns.SimulationEngine.from_spec(spec) as eng:
+        fit = surrogate.fit(eng)
+        params, norm = eng.surrogate_params()
+        res["fit"] = {"samples": fit["samples"], "generate_s": fit["generate_s"],
+                      "train_s": fit["train_s"], "steps": len(fit["loss"]),
+                      "final_loss": float(fit["loss"][-200:].mean())}
+        steps = int(round(sim_ms / 0.05))
+        wall, st = _timed(eng, steps, graph)
+        res["surrogate"] = {"dt_ms": 0.05, "ms_per_sim_s": wall / sim_ms * 1e6,
+                            "us_per_step": wall / steps * 1e6,
+                            "rate_hz": st.spikes / n / (sim_ms / 1e3),
+                            "fallback_neuron_steps": eng.surrogate_fallbacks()}
+        eng.set_profiling(True)
+        eng.step(200)
+        ms, launches, _ = eng.kernel_time(0)
+        eng.set_profiling(False)
+        us = ms / max(launches, 1) * 1e3
+        flop = 2.0 * (5 * 64 + 64 * 64 + 2 * 64 * 4) * n  # W3 runs as two bf16 words
+        res["surrogate"]["update_kernel_us"] = us
+        res["surrogate"]["mlp_tflops"] = flop / (us * 1e-6) / 1e12
+        res["surrogate"]["mlp_frac_of_bf16_peak"] = flop / (us * 1e-6) / 2.5e15
+    for name, dt, sub in [("plain_dt0.01", 0.01, 1), ("plain_dt0.05_x5", 0.05, 5)]:
+        with ns.SimulationEngine.from_spec(configs.hh_network(n=n, dt_ms=dt,
+                                                              hh_substeps=sub)) as eng:
+            steps = int(round(sim_ms / dt))
+            wall, st = _timed(eng, steps, graph)
+            res[name] = {"dt_ms": dt, "substeps": sub, "ms_per_sim_s": wall / sim_ms * 1e6,
+                         "us_per_step": wall / steps * 1e6,
+                         "rate_hz": st.spikes / n / (sim_ms / 1e3)}
+    s_ms = res["surrogate"]["ms_per_sim_s"]
+    res["speedup_vs_plain_dt0.01"] = res["plain_dt0.01"]["ms_per_sim_s"] / s_ms
+    res["speedup_vs_plain_dt0.05_x5"] = res["plain_dt0.05_x5"]["ms_per_sim_s"] / s_ms
+    # accuracy: 2000 uncoupled neurons, 500 ms, same OU drive
+    kw = dict(k_in=1, w_exc=(0.0, 0.0), w_inh=(0.0, 0.0), record_raster=True,
+              raster_capacity=1 << 22)
+    u = configs.hh_surrogate_network(n=2000, **kw)
+    with ns.SimulationEngine.from_spec(u) as eng:
+        eng.set_surrogate_params(params, norm)
+        eng.step(10_000)
+        a = eng.raster()
+    with ns.SimulationEngine.from_spec(configs.hh_network(n=2000, dt_ms=0.05, hh_substeps=5,
+                                                          **kw)) as eng:
+        eng.step(10_000)
+        b = eng.raster()
+    res["displaced_vs_plain_fp32"] = spike_displacement(b[0], b[1], a[0], a[1], 0.05, 0.5)
+    return res
+
+
+def bench_population(ns, steps=2000, tune_steps=50):
+    """256 Izhikevich networks (N log-uniform in [1k, 64k], in-degree uniform in [20, 200]) as
+    one batched population, 2 s at dt 1 ms after autotuning; RateHomeostasis(5 Hz) fitness."""
+    from neuralsim_amd import configs
+    from neuralsim_amd.population import RateHomeostasis
+    import numpy as np
+    base = configs.population_base()
+    nets = configs.population_nets(256)
+    t0 = time.perf_counter()
+    with ns.BatchedPopulation(base, nets) as pop:
+        build_s = time.perf_counter() - t0
+        split, graph, trials = pop.autotune(tune_steps)
+        t0 = time.perf_counter()
+        st = pop.step(steps)
+        wall = time.perf_counter() - t0
+        spikes, _ = pop.net_stats()
+    n = np.array([b.n_neurons for b in nets], np.float64)
+    rate = spikes / n / (steps * base.dt_ms / 1e3)
+    fit = RateHomeostasis(5.0).fitness(rate)
+    return {"networks": len(nets), "neurons": int(n.sum()),
+            "synapses": int(sum(b.n_neurons * b.k_in for b in nets)), "steps": steps,
+            "build_s": build_s, "tuned": {"prop_split": split, "graph_steps": graph},
+            "ms_per_step": wall * 1e3 / steps, "x_realtime": steps * base.dt_ms / 1e3 / wall,
+            "syn_events_per_s": st.events / wall, "spikes": int(st.spikes),
+            "rate_hz": {"min": float(rate.min()), "median": float(np.median(rate)),
+                        "max": float(rate.max())},
+            "fitness_mean": float(fit.mean()), "fitness_best": float(fit.max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +183,8 @@ def main():
     ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hh", action="store_true", help="skip the HH surrogate comparison")
+    ap.add_argument("--no-pop", action="store_true", help="skip the batched population")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -192,9 +288,17 @@ def main():
                 "syn_events_per_s": gst.events / g_wall,
                 "x_realtime": g_steps * spec.cfg.dt_ms / 1000.0 / g_wall,
             }
+    eng.close()
+    if not args.no_hh:
+        log("bench: HH surrogate vs plain (200k neurons)")
+        out["hh_surrogate"] = bench_hh(ns)
+        log(f"bench: HH speedup vs plain dt 0.01 ms {out['hh_surrogate']['speedup_vs_plain_dt0.01']:.2f}")
+    if not args.no_pop:
+        log("bench: batched population (256 networks)")
+        out["population"] = bench_population(ns)
+        log(f"bench: population {out['population']['ms_per_step']:.3f} ms/step")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, events / args.steps, args.cpu_steps)
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

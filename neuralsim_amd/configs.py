@@ -194,14 +194,14 @@ def _mix64(x):
 def population_base(record_raster=False, raster_capacity=0):
     """Shared neuron/drive settings of the batched Izhikevich population: RS/FS 80/20 with the
     c/d jitter of the 1M workload, dt 1 ms with two 0.5 ms V sub-steps, delays 1-10 ms (11 ring
-    slots), and a 5 Hz Poisson drive of 100 inputs x 10 (current units) per network (the
+    slots), and a 5 Hz Poisson drive of 100 inputs x 4 (current units) per network (the
     workload leaves the drive's size open)."""
     c = izh_stdp_network(n=2, plasticity=False).cfg
     c.ring_slots = 11
     c.drive = abi.DRIVE_POISSON
     c.drive_inputs = 100
     c.drive_rate_hz = 5.0
-    c.drive_weight = 10.0
+    c.drive_weight = 4.0
     c.drive_mean = 0.0
     c.drive_sigma = 0.0
     c.record_raster = 1 if record_raster else 0
@@ -210,7 +210,7 @@ def population_base(record_raster=False, raster_capacity=0):
 
 
 def population_nets(n_nets=256, seed=2026, n_range=(1000, 64000), k_range=(20, 200),
-                    w_exc=(0.0, 6.0), w_inh=(-5.0, 0.0), delays=(1, 10)):
+                    w_exc=(0.0, 3.0), w_inh=(-5.0, 0.0), delays=(1, 10)):
     """Networks of the batched population: N log-uniform in n_range, in-degree uniform in
     k_range, 80/20 E/I, own seed each (SplitMix64 of the master seed and the index)."""
     import numpy as np

@@ -179,7 +179,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--graph", type=int, default=50, help="steps per captured hipGraph (0: off)")
-    ap.add_argument("--no-profile", action="store_true", help="no per-kernel events")
+    ap.add_argument("--no-profile", action="store_true", help="no per-kernel event segment")
+    ap.add_argument("--profile-steps", type=int, default=2000,
+                    help="steps of the direct-launch segment timed per kernel")
     ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
@@ -200,14 +202,14 @@ def main():
     log(f"bench: {spec.name} n={spec.n} m={spec.m} built on device in {build_s:.2f}s")
     if args.split:
         eng.set_prop_split(args.split)
-    # per-kernel hipEvents bracket direct launches on the library's stream; without them the
-    # timed region replays captured graphs of --graph steps
+    # the timed region replays captured graphs of --graph steps (the production path); a
+    # separate segment of direct launches bracketed by per-kernel hipEvents on the library's
+    # stream gives the kernel durations
     prof = not args.no_profile
-    if args.graph and not prof:
+    if args.graph:
         eng.set_graph_steps(args.graph)
     eng.step(args.warmup)
     log(f"bench: warmup {args.warmup} steps done")
-    eng.set_profiling(prof)
     t0 = time.perf_counter()
     st = eng.step(args.steps)
     wall = time.perf_counter() - t0
@@ -233,7 +235,7 @@ def main():
             "neurons": spec.n, "synapses": spec.m, "dt_ms": spec.cfg.dt_ms,
             "v_substeps": spec.cfg.v_substeps, "delays_ms": [1, 20], "sim_ms": sim_s * 1e3,
             "accumulate": "int64 fixed point 2^-32",
-            "launch": "direct, hipEvents per kernel" if prof else f"hipGraph x{args.graph}",
+            "launch": f"hipGraph x{args.graph}" if args.graph else "direct",
         },
         "x_realtime": sim_s / wall,
         "spikes": spikes,
@@ -242,23 +244,29 @@ def main():
         "build_s": build_s,
     }
     if prof:
+        p_steps = min(args.steps, args.profile_steps)
+        eng.set_graph_steps(0)
+        eng.set_profiling(True)
+        pst = eng.step(p_steps)
+        eng.set_profiling(False)
+        p_events, p_spikes = pst.events, pst.spikes
         kinds = {0: "k_update", 1: "k_synapses"}
         times = {k: eng.kernel_time(k) for k in kinds}
         ltp_slots, ltd_slots = times[1][2], eng.kernel_time(2)[2]
-        per = 1.0 / args.steps
-        upd_b = neuron_bytes(spec.cfg, spec.n, spikes * per)
+        per = 1.0 / p_steps
+        upd_b = neuron_bytes(spec.cfg, spec.n, p_spikes * per)
         # propagation: 4 B col + 4 B weight + 1 B delay + 8 B int64 accumulate per event and
         # 16 B of row pointers per spiking row; potentiation: 16 B post record + 4 B weight
         # write per E->E out-slot; depression: 4 B slot + 4 B source + 8 B pre record + 8 B
         # weight read+write per E->E in-slot
-        prop_b = (17.0 * events + 16.0 * spikes) * per
+        prop_b = (17.0 * p_events + 16.0 * p_spikes) * per
         syn_b = prop_b + (20.0 * ltp_slots + 24.0 * ltd_slots) * per
         kt = {kinds[k]: {"avg_us": (ms / max(n, 1)) * 1e3, "launches": n}
               for k, (ms, n, _) in times.items()}
         kt["k_update"]["alg_bytes_per_launch"] = upd_b
         kt["k_synapses"]["alg_bytes_per_launch"] = syn_b
         kt["k_synapses"]["propagation_bytes_per_launch"] = prop_b
-        kt["k_synapses"]["events_per_launch"] = events * per
+        kt["k_synapses"]["events_per_launch"] = p_events * per
         kt["k_synapses"]["ltp_slots_per_launch"] = ltp_slots * per
         kt["k_synapses"]["ltd_slots_per_launch"] = ltd_slots * per
         for k in kinds.values():
@@ -266,7 +274,7 @@ def main():
             kt[k]["achieved_GBs"] = a
         # the synapse pass issues one 64-bit atomic per event: rate vs the measured random
         # atomic throughput of the device (23.0 G/s, int64 add to random addresses)
-        kt["k_synapses"]["atomics_G_per_s"] = events * per / (kt["k_synapses"]["avg_us"] * 1e3)
+        kt["k_synapses"]["atomics_G_per_s"] = p_events * per / (kt["k_synapses"]["avg_us"] * 1e3)
         kt["k_synapses"]["atomic_ceiling_G_per_s"] = 23.0
         dom = max(kinds.values(), key=lambda k: kt[k]["avg_us"])
         out["roofline"] = {
@@ -275,20 +283,8 @@ def main():
             "traffic": measured_traffic(dom),
         }
         out["kernels"] = kt
-        if args.graph:
-            # the same network continued with graph replay and no per-kernel events
-            eng.set_profiling(False)
-            eng.set_graph_steps(args.graph)
-            g_steps = min(args.steps, 3000)
-            t0 = time.perf_counter()
-            gst = eng.step(g_steps)
-            g_wall = time.perf_counter() - t0
-            out["graph_replay"] = {
-                "steps": g_steps, "graph_steps": args.graph, "ms_per_step": g_wall * 1e3 / g_steps,
-                "syn_events_per_s": gst.events / g_wall,
-                "x_realtime": g_steps * spec.cfg.dt_ms / 1000.0 / g_wall,
-            }
-    eng.close()
+        out["profiled_segment"] = {"steps": p_steps, "launch": "direct, hipEvents per kernel",
+                                   "ms_per_step": pst.wall_ms / p_steps}
     if not args.no_hh:
         log("bench: HH surrogate vs plain (200k neurons)")
         out["hh_surrogate"] = bench_hh(ns)

@@ -66,7 +66,12 @@ typedef enum nsx_drive_kind {
 typedef enum nsx_plasticity_kind {
   NSX_PLAST_NONE = 0,
   NSX_PLAST_PAIR = 1,   /* StdpRule + StdpTrace as engine.rs:496-537 applies them, E->E only */
-  NSX_PLAST_TRIPLET = 2 /* online triplet rule (Pfister & Gerstner 2006), E->E only          */
+  NSX_PLAST_TRIPLET = 2 /* TripletStdpRule (plasticity.rs:48-81) applied online, E->E only:
+                         * at a pre spike w -= a_minus e^(-(t - t_post)/tau-) with t_post the
+                         * post neuron's last spike (same step included); at a post spike
+                         * w += a_plus e^(-(t - t_pre)/tau+) + a3_plus e^(-(t - t_post')/tau_y)
+                         * with t_pre the pre neuron's last spike before t and t_post' the post
+                         * neuron's previous spike: the closed form on nearest-spike pairs */
 } nsx_plasticity_kind;
 
 /* CROSSING: standard HH, the membrane equation of hodgkin_huxley.rs:6-9, spike = upward

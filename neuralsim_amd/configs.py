@@ -107,8 +107,9 @@ def lif_network(n=10_000, seed=1, drive_scale=1.0, steps=10_000, plasticity=Fals
 
 
 def izh_stdp_network(n=1_000_000, seed=7, steps=10_000, plasticity=True, record_raster=False,
-                     raster_capacity=0, k_in=100):
-    """Izhikevich + pair STDP: 80% RS / 20% FS, in-degree 100, delays 1-20 ms, N(0,5) per ms."""
+                     raster_capacity=0, k_in=100, rule="pair"):
+    """Izhikevich + pair STDP: 80% RS / 20% FS, in-degree 100, delays 1-20 ms, N(0,5) per ms.
+    rule="triplet" swaps in triplet STDP with TripletStdpRule's defaults (plasticity.rs:59-71)."""
     c = base_config(abi.MODEL_IZHIKEVICH)
     c.n_neurons = n
     c.n_exc = (n * 4) // 5
@@ -126,6 +127,10 @@ def izh_stdp_network(n=1_000_000, seed=7, steps=10_000, plasticity=True, record_
     c.plasticity = abi.PLAST_PAIR if plasticity else abi.PLAST_NONE
     c.a_plus, c.a_minus, c.tau_plus_ms, c.tau_minus_ms = 0.01, 0.012, 20.0, 20.0
     c.w_min, c.w_max = 0.0, 10.0
+    if plasticity and rule == "triplet":
+        c.plasticity = abi.PLAST_TRIPLET
+        c.tau_plus_ms, c.tau_minus_ms, c.tau_y_ms = 16.8, 33.7, 100.0
+        c.a_plus, c.a_minus, c.a3_plus = 5e-4, 2e-4, 3e-4
     c.record_raster = 1 if record_raster else 0
     c.raster_capacity = raster_capacity
     return NetSpec(f"izh_stdp_{n}", c, k_in, (0.0, 6.0), (-5.0, 0.0), (1, 20), steps,

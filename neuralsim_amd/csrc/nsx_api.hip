@@ -138,12 +138,17 @@ static nsx_status validate_config(const nsx_config* c) {
   if (c->frac_bits > 40) return set_error(nullptr, NSX_E_INVALID, "frac_bits must be <= 40");
   if (c->v_substeps < 1) return set_error(nullptr, NSX_E_INVALID, "v_substeps must be >= 1");
   if (c->drive > NSX_DRIVE_OU) return set_error(nullptr, NSX_E_INVALID, "unknown drive");
-  if (c->plasticity > NSX_PLAST_PAIR)
+  if (c->plasticity > NSX_PLAST_TRIPLET)
     return set_error(nullptr, NSX_E_INVALID, "unsupported plasticity kind");
-  if (c->plasticity == NSX_PLAST_PAIR &&
+  if (c->plasticity != NSX_PLAST_NONE &&
       (!(c->tau_plus_ms > 0) || !(c->tau_minus_ms > 0) ||
        104.0 * std::max(c->tau_plus_ms, c->tau_minus_ms) / c->dt_ms > (double)(1 << 24)))
     return set_error(nullptr, NSX_E_INVALID, "STDP time constants out of range");
+  if (c->plasticity == NSX_PLAST_TRIPLET &&
+      (!(c->tau_y_ms > 0) || 104.0 * c->tau_y_ms / c->dt_ms > (double)(1 << 24)))
+    return set_error(nullptr, NSX_E_INVALID, "triplet tau_y out of range");
+  if (c->plasticity == NSX_PLAST_TRIPLET && c->model == NSX_MODEL_HH_SURROGATE)
+    return set_error(nullptr, NSX_E_INVALID, "the HH surrogate runs with pair STDP or none");
   if (c->drive == NSX_DRIVE_OU && !(c->drive_tau_ms > 0))
     return set_error(nullptr, NSX_E_INVALID, "drive_tau_ms must be > 0");
   if (c->drive == NSX_DRIVE_POISSON) {
@@ -218,6 +223,7 @@ static void fill_params(nsx_ctx* ctx) {
                             kMaxPoisson);
   p.a_plus = c.a_plus;
   p.a_minus = c.a_minus;
+  p.a3 = c.a3_plus;
   p.w_min = c.w_min;
   p.w_max = c.w_max;
   p.step_counts_cap = 1u << 20;
@@ -404,13 +410,18 @@ nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out) {
     if ((s = dev_alloc(ctx, (void**)&d.raster_step, cfg->raster_capacity * 4, false))) goto fail;
     if ((s = dev_alloc(ctx, (void**)&d.raster_id, cfg->raster_capacity * 4, false))) goto fail;
   }
-  if (cfg->plasticity == NSX_PLAST_PAIR) {
+  if (cfg->plasticity != NSX_PLAST_NONE) {
     float* pt = nullptr;
     float* mt = nullptr;
     if ((s = make_trace_table(ctx, cfg->tau_plus_ms, &pt, &ctx->p.ptab_n, ctx->ptab_h))) goto fail;
     if ((s = make_trace_table(ctx, cfg->tau_minus_ms, &mt, &ctx->p.mtab_n, ctx->mtab_h))) goto fail;
     d.ptab = pt;
     d.mtab = mt;
+  }
+  if (cfg->plasticity == NSX_PLAST_TRIPLET) {
+    float* yt = nullptr;
+    if ((s = make_trace_table(ctx, cfg->tau_y_ms, &yt, &ctx->p.ytab_n, ctx->ytab_h))) goto fail;
+    d.ytab = yt;
   }
   if (cfg->model == NSX_MODEL_HH_SURROGATE &&
       (s = dev_alloc(ctx, (void**)&ctx->sur.fb, (size_t)ctx->p.nblk * 8, true)))

@@ -230,7 +230,7 @@ nsx_status build_csr_device(nsx_ctx* ctx, uint32_t* d_src, uint32_t* d_dst, floa
   dev_free(ctx, keys_out);
   dev_free(ctx, perm_out);
   ctx->m = m;
-  if (ctx->cfg.plasticity == NSX_PLAST_PAIR && (s = build_csc_device(ctx))) return s;
+  if (ctx->cfg.plasticity != NSX_PLAST_NONE && (s = build_csc_device(ctx))) return s;
   publish_network(ctx);
   return NSX_OK;
 }
@@ -350,7 +350,7 @@ nsx_status nsx_upload_csr(nsx_ctx* ctx, const uint64_t* row_ptr, const uint32_t*
     free_network(ctx);
     return s;
   }
-  if (ctx->cfg.plasticity == NSX_PLAST_PAIR && (s = build_csc_device(ctx))) return s;
+  if (ctx->cfg.plasticity != NSX_PLAST_NONE && (s = build_csc_device(ctx))) return s;
   publish_network(ctx);
   return NSX_OK;
 }

@@ -20,10 +20,11 @@ struct __attribute__((aligned(8))) TraceRec {
   int32_t s;
 };
 
-// Post-synaptic trace record: the lazily decayed trace, plus the step of the neuron's last spike
+// Post-synaptic record: the lazily decayed trace {v, s}, plus the step of the neuron's last spike
 // so that the potentiation pass sees same-step post spikes in the same cache line.  The trace
 // bump of a spike at step t is applied by the update kernel of step t+1 (after every pass of
-// step t has read the pre-bump value).
+// step t has read the pre-bump value).  Triplet STDP keeps no trace: s holds the step of the
+// neuron's spike before `last`.
 struct __attribute__((aligned(16))) PostRec {
   float v;
   int32_t s;
@@ -76,9 +77,9 @@ struct StepParams {
   float drive_mean, drive_sigma, drive_weight, ou_k1, ou_k2, ou_k3;
   uint32_t pthr[kMaxPoisson];
   int32_t npthr;
-  // pair STDP
-  float a_plus, a_minus, w_min, w_max;
-  int32_t ptab_n, mtab_n;
+  // STDP (pair: traces; triplet: nearest-spike intervals through the same tables)
+  float a_plus, a_minus, w_min, w_max, a3;
+  int32_t ptab_n, mtab_n, ytab_n;
   uint32_t step_counts_cap;
   uint32_t raster_on;
   uint64_t raster_cap;
@@ -94,7 +95,7 @@ struct DevPtrs {
   TraceRec* pre;
   PostRec* post;
   long long* ring;
-  const float *ptab, *mtab;
+  const float *ptab, *mtab, *ytab;  // exp(-k dt / tau) for tau+, tau- and (triplet) tau_y
   uint32_t* spikes;  // per-block spike segments: block b's spikes at [b*kBlock, +bcnt[b])
   uint32_t* bcnt;    // spikes of each neuron block in the current step
   unsigned long long* part;  // per-block running totals, kPart* planes of nblk entries
@@ -180,7 +181,7 @@ struct nsx_ctx {
   uint32_t* csc_slot = nullptr;
   uint32_t* csc_src = nullptr;
   uint64_t csc_m = 0;
-  std::vector<float> ptab_h, mtab_h;  // host copies of the trace decay tables
+  std::vector<float> ptab_h, mtab_h, ytab_h;  // host copies of the trace decay tables
   // graph
   uint32_t graph_steps = 0;
   hipGraphExec_t graph_exec = nullptr;

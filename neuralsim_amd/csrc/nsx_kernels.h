@@ -109,16 +109,23 @@ __device__ inline void ring_take(const StepParams& p, const DevPtrs& s, uint32_t
   }
 }
 
-// Bookkeeping of a spike of neuron i at step t: count, last spike step, pre-synaptic trace bump
-// and the post record's same-step flag (excitatory neurons, pair STDP).
-template <bool PAIR>
+// Bookkeeping of a spike of neuron i at step t: count, last spike step; for excitatory neurons
+// under pair STDP the pre-synaptic trace bump and the post record's same-step flag, under
+// triplet STDP the post record's (previous, last) spike steps.  PL is an nsx_plasticity_kind.
+template <int PL>
 __device__ inline void spike_record(const StepParams& p, const DevPtrs& s, uint32_t i,
                                     int32_t t) {
   s.count[i] += 1u;
   s.last[i] = t;
-  if (PAIR && i < p.n_exc) {
+  if (PL == NSX_PLAST_PAIR && i < p.n_exc) {
     s.pre[i] = trace_bump(s.pre[i], s.ptab, p.ptab_n, t);
     s.post[i].last = t;
+  }
+  if (PL == NSX_PLAST_TRIPLET && i < p.n_exc) {
+    PostRec r = s.post[i];
+    r.s = r.last;
+    r.last = t;
+    s.post[i] = r;
   }
 }
 

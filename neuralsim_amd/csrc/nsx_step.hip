@@ -10,10 +10,12 @@
 //                * propagation: a neuron block's spiking rows flattened across the 64 lanes
 //                  (coalesced col / weight / delay streams); each slot adds rint(w 2^S) into
 //                  ring[(t+d) mod D][channel][target] with a 64-bit integer atomic; E->E slots
-//                  get pair-STDP potentiation, then depression when the target fired at t too
+//                  get STDP at the pre spike (pair: potentiation by the post trace, then
+//                  depression when the target fired at t too; triplet: depression)
 //                  (engine.rs:414-473 with delays, engine.rs:512-534; spmv.wgsl);
-//                * depression: E->E in-slots (transpose index) of a block's spiking excitatory
-//                  neurons whose source did not fire at t (engine.rs:524-534).
+//                * post-spike: E->E in-slots (transpose index) of a block's spiking excitatory
+//                  neurons whose source did not fire at t (pair: depression, engine.rs:524-534;
+//                  triplet: potentiation).
 //                Every weight has one writer per step, so the kinds run concurrently: the
 //                atomic-throughput-bound propagation overlaps the latency-bound depression.
 // Kernels read the step index as *tbase + off so that a captured hipGraph of G steps can be
@@ -25,11 +27,11 @@
 
 namespace nsx {
 
-template <int MODEL, int DRIVE, bool PAIR>
+template <int MODEL, int DRIVE, int PL>
 __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
   const int32_t t = *s.tbase + off;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  update_prologue<PAIR>(p, s, t);
+  update_prologue<PL == NSX_PLAST_PAIR>(p, s, t);
   const NeuronId id = neuron_id(p, s, i, blockIdx.x);
   bool spiked = false;
   if (i < p.n && id.local < id.n) {  // batch padding neurons stay idle
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       s.gh[i] = h;
       s.gn[i] = nn;
     }
-    if (spiked) spike_record<PAIR>(p, s, i, t);
+    if (spiked) spike_record<PL>(p, s, i, t);
   }
   block_compact(p, s, spiked, i, t, blockIdx.x);
 }
@@ -159,8 +161,10 @@ __device__ inline uint64_t chunk_slot(const RowChunk& rc, uint32_t f, uint32_t& 
   return rc.beg[r] + (f - excl);
 }
 
-// Propagation work item: part q of neuron block b.
-template <bool PAIR>
+// Propagation work item: part q of neuron block b.  PL: STDP applied on the spiking rows' E->E
+// slots -- pair: potentiation by the post trace (then depression if the target fired at t);
+// triplet: depression by the target's last spike.
+template <int PL>
 __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
                                uint32_t b, uint32_t q, uint32_t lane) {
   const uint32_t P = p.prop_split;
@@ -193,8 +197,8 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
       const uint32_t j0 = s.col[k0], j1 = s.col[k1];
       const float w0 = s.w[k0], w1 = s.w[k1];
       const uint32_t d0 = s.delay[k0], d1 = s.delay[k1];
-      const bool pot0 = PAIR && src0 < p.n_exc && j0 < p.n_exc;
-      const bool pot1 = PAIR && two && src1 < p.n_exc && j1 < p.n_exc;
+      const bool pot0 = PL != NSX_PLAST_NONE && src0 < p.n_exc && j0 < p.n_exc;
+      const bool pot1 = PL != NSX_PLAST_NONE && two && src1 < p.n_exc && j1 < p.n_exc;
       PostRec pr0{0.0f, kNeverStep, kNeverStep, 0}, pr1{0.0f, kNeverStep, kNeverStep, 0};
       if (pot0) pr0 = s.post[j0];
       if (pot1) pr1 = s.post[j1];
@@ -208,7 +212,24 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
       if (two)
         atomicAdd((unsigned long long*)(s.ring + sl1 * plane + (size_t)ch1 * p.n + j1),
                   (unsigned long long)quantize(w1, p.scale_in));
-      if (PAIR && (pot0 || pot1)) {
+      if (PL == NSX_PLAST_TRIPLET && (pot0 || pot1)) {
+        // pre spike at t: depression by the target's last spike (same step included)
+        if (pot0) {
+          const float x = trace_at(s.mtab, p.mtab_n, t - pr0.last);
+          const float dlt = p.a_minus * x;
+          const float wn = w0 - dlt;
+          s.w[k0] = clampf_(wn, p.w_min, p.w_max);
+          ltp += 1;
+        }
+        if (pot1) {
+          const float x = trace_at(s.mtab, p.mtab_n, t - pr1.last);
+          const float dlt = p.a_minus * x;
+          const float wn = w1 - dlt;
+          s.w[k1] = clampf_(wn, p.w_min, p.w_max);
+          ltp += 1;
+        }
+      }
+      if (PL == NSX_PLAST_PAIR && (pot0 || pot1)) {
         // the source fired at t, so its pre trace is the record bumped at t (factor P[0] = 1)
         if (pot0) {
           const float x = trace_now(pr0.v, pr0.s, s.mtab, p.mtab_n, t);
@@ -241,7 +262,7 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
     }
     wave_sync();
   }
-  if (PAIR) {
+  if (PL != NSX_PLAST_NONE) {
 #pragma unroll
     for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
   }
@@ -249,15 +270,18 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
     // parts of one block accumulate into that block's counter
     if (P == 1) {
       s.part[kPartEvents * p.nblk + b] += ev;
-      if (PAIR) s.part[kPartLtp * p.nblk + b] += ltp;
+      if (PL != NSX_PLAST_NONE) s.part[kPartLtp * p.nblk + b] += ltp;
     } else {
       atomicAdd(&s.part[kPartEvents * p.nblk + b], ev);
-      if (PAIR) atomicAdd(&s.part[kPartLtp * p.nblk + b], ltp);
+      if (PL != NSX_PLAST_NONE) atomicAdd(&s.part[kPartLtp * p.nblk + b], ltp);
     }
   }
 }
 
-// Depression work item: part q of neuron block b (excitatory neurons only).
+// Post-spike work item: part q of neuron block b (excitatory neurons only), over the E->E
+// in-slots of its spiking neurons whose source did not fire at t -- pair: depression by the pre
+// trace; triplet: potentiation by the source's last spike and the target's previous spike.
+template <int PL>
 __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
                              uint32_t b, uint32_t q, uint32_t lane) {
   const uint32_t P = p.prop_split;
@@ -286,6 +310,38 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
       const uint64_t e1 = two ? chunk_slot(rc, f + 64, r1) : e0;
       const uint32_t i0 = s.csc_src[e0], i1 = s.csc_src[e1];
       const uint32_t k0 = s.csc_slot[e0], k1 = s.csc_slot[e1];
+      if (PL == NSX_PLAST_TRIPLET) {
+        const int32_t l0 = s.last[i0], l1 = s.last[i1];
+        // the source's propagation item owns same-step pairs; a source that never fired
+        // pairs with nothing
+        const bool own0 = l0 != t && l0 != kNeverStep;
+        const bool own1 = two && l1 != t && l1 != kNeverStep;
+        float w0 = 0.0f, w1 = 0.0f;
+        if (own0) w0 = s.w[k0];
+        if (own1) w1 = s.w[k1];
+        if (own0) {
+          const int32_t pv = s.post[rc.src[r]].s;
+          const float x1 = trace_at(s.ptab, p.ptab_n, t - l0);
+          const float x2 = trace_at(s.ytab, p.ytab_n, t - pv);
+          float dw = p.a_plus * x1;
+          const float y = p.a3 * x2;
+          dw = dw + y;
+          const float wn = w0 + dw;
+          s.w[k0] = clampf_(wn, p.w_min, p.w_max);
+        }
+        if (own1) {
+          const int32_t pv = s.post[rc.src[r1]].s;
+          const float x1 = trace_at(s.ptab, p.ptab_n, t - l1);
+          const float x2 = trace_at(s.ytab, p.ytab_n, t - pv);
+          float dw = p.a_plus * x1;
+          const float y = p.a3 * x2;
+          dw = dw + y;
+          const float wn = w1 + dw;
+          s.w[k1] = clampf_(wn, p.w_min, p.w_max);
+        }
+        r = r1;
+        continue;
+      }
       const TraceRec a0 = s.pre[i0], a1 = s.pre[i1];
       // a source that fired at t owns this slot in its propagation item
       const bool own0 = a0.s != t, own1 = two && a1.s != t;
@@ -318,7 +374,7 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
 
 // Work items: propagation (nblk * P) and depression (nblk_e * P), interleaved so that both
 // kinds are resident on every CU.
-template <bool PAIR>
+template <int PL>
 __global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, int32_t off) {
   __shared__ RowChunk sh[kBlock / 64];
   const int32_t t = *s.tbase + off;
@@ -326,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, in
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
   const uint32_t P = p.prop_split;
   const uint32_t nprop = p.nblk * P;
-  const uint32_t nltd = PAIR ? ((p.n_exc + kBlock - 1) / kBlock) * P : 0u;
+  const uint32_t nltd = PL != NSX_PLAST_NONE ? ((p.n_exc + kBlock - 1) / kBlock) * P : 0u;
   const uint32_t both = 2 * min(nprop, nltd);
   const uint32_t nwork = nprop + nltd;
   const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
@@ -342,9 +398,9 @@ __global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, in
     }
     const uint32_t b = item / P, q = item - b * P;
     if (prop)
-      propagate_item<PAIR>(p, s, sh[wid], t, b, q, lane);
-    else if (PAIR)
-      depress_item(p, s, sh[wid], t, b, q, lane);
+      propagate_item<PL>(p, s, sh[wid], t, b, q, lane);
+    else if (PL != NSX_PLAST_NONE)
+      depress_item<PL>(p, s, sh[wid], t, b, q, lane);
   }
 }
 
@@ -354,29 +410,33 @@ __global__ void k_advance(int32_t* tbase, int32_t g) { *tbase += g; }
 typedef void (*UpdateFn)(StepParams, DevPtrs, int32_t);
 
 template <int MODEL, int DRIVE>
-static UpdateFn pick_pair(bool pair) {
-  return pair ? k_update<MODEL, DRIVE, true> : k_update<MODEL, DRIVE, false>;
-}
-template <int MODEL>
-static UpdateFn pick_drive(int drive, bool pair) {
-  switch (drive) {
-    case NSX_DRIVE_POISSON: return pick_pair<MODEL, NSX_DRIVE_POISSON>(pair);
-    case NSX_DRIVE_GAUSSIAN: return pick_pair<MODEL, NSX_DRIVE_GAUSSIAN>(pair);
-    case NSX_DRIVE_OU: return pick_pair<MODEL, NSX_DRIVE_OU>(pair);
-    default: return pick_pair<MODEL, NSX_DRIVE_NONE>(pair);
+static UpdateFn pick_plast(int pl) {
+  switch (pl) {
+    case NSX_PLAST_PAIR: return k_update<MODEL, DRIVE, NSX_PLAST_PAIR>;
+    case NSX_PLAST_TRIPLET: return k_update<MODEL, DRIVE, NSX_PLAST_TRIPLET>;
+    default: return k_update<MODEL, DRIVE, NSX_PLAST_NONE>;
   }
 }
-UpdateFn pick_update(int model, int drive, bool pair) {
+template <int MODEL>
+static UpdateFn pick_drive(int drive, int pl) {
+  switch (drive) {
+    case NSX_DRIVE_POISSON: return pick_plast<MODEL, NSX_DRIVE_POISSON>(pl);
+    case NSX_DRIVE_GAUSSIAN: return pick_plast<MODEL, NSX_DRIVE_GAUSSIAN>(pl);
+    case NSX_DRIVE_OU: return pick_plast<MODEL, NSX_DRIVE_OU>(pl);
+    default: return pick_plast<MODEL, NSX_DRIVE_NONE>(pl);
+  }
+}
+UpdateFn pick_update(int model, int drive, int pl) {
   switch (model) {
-    case NSX_MODEL_LIF: return pick_drive<NSX_MODEL_LIF>(drive, pair);
-    case NSX_MODEL_IZHIKEVICH: return pick_drive<NSX_MODEL_IZHIKEVICH>(drive, pair);
-    default: return pick_drive<NSX_MODEL_HH>(drive, pair);
+    case NSX_MODEL_LIF: return pick_drive<NSX_MODEL_LIF>(drive, pl);
+    case NSX_MODEL_IZHIKEVICH: return pick_drive<NSX_MODEL_IZHIKEVICH>(drive, pl);
+    default: return pick_drive<NSX_MODEL_HH>(drive, pl);
   }
 }
 
 // one wavefront per work item
 uint32_t syn_grid(const nsx_ctx* ctx) {
-  const bool pair = ctx->cfg.plasticity == NSX_PLAST_PAIR;
+  const bool pair = ctx->cfg.plasticity != NSX_PLAST_NONE;
   const uint64_t nblk_e = (ctx->cfg.n_exc + kBlock - 1) / kBlock;
   const uint64_t waves = ((uint64_t)ctx->p.nblk + (pair ? nblk_e : 0)) * ctx->p.prop_split;
   return (uint32_t)((waves + kBlock / 64 - 1) / (kBlock / 64));
@@ -385,8 +445,8 @@ uint32_t syn_grid(const nsx_ctx* ctx) {
 // Enqueue the launches of one step with offset `off` from *tbase.
 hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
   const StepParams& p = ctx->p;
-  const bool pair = ctx->cfg.plasticity == NSX_PLAST_PAIR;
-  UpdateFn upd = pick_update((int)ctx->cfg.model, (int)ctx->cfg.drive, pair);
+  const int pl = (int)ctx->cfg.plasticity;
+  UpdateFn upd = pick_update((int)ctx->cfg.model, (int)ctx->cfg.drive, pl);
   const uint32_t gu = (uint32_t)((ctx->n + kBlock - 1) / kBlock);
   const uint32_t gs = syn_grid(ctx);
   if (evs) (void)hipEventRecord(evs[0], ctx->stream);
@@ -397,10 +457,15 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
     hipLaunchKernelGGL(upd, dim3(gu), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
   }
   if (evs) (void)hipEventRecord(evs[1], ctx->stream);
-  if (pair)
-    hipLaunchKernelGGL(k_synapses<true>, dim3(gs), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+  if (pl == NSX_PLAST_PAIR)
+    hipLaunchKernelGGL(k_synapses<NSX_PLAST_PAIR>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
+                       ctx->d, off);
+  else if (pl == NSX_PLAST_TRIPLET)
+    hipLaunchKernelGGL(k_synapses<NSX_PLAST_TRIPLET>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
+                       ctx->d, off);
   else
-    hipLaunchKernelGGL(k_synapses<false>, dim3(gs), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
+    hipLaunchKernelGGL(k_synapses<NSX_PLAST_NONE>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
+                       ctx->d, off);
   if (evs) (void)hipEventRecord(evs[2], ctx->stream);
   return hipGetLastError();
 }

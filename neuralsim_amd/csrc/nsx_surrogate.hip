@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
     s.gm[i] = nx[1];
     s.gh[i] = nx[2];
     s.gn[i] = nx[3];
-    if (spiked) spike_record<PAIR>(p, s, i, t);
+    if (spiked) spike_record<PAIR ? NSX_PLAST_PAIR : NSX_PLAST_NONE>(p, s, i, t);
   }
   const unsigned long long fbm = __ballot(live && !in);
   if (lane == 0) s_fb[wid] = (uint32_t)__popcll(fbm);

@@ -644,8 +644,8 @@ struct mir_net {
   uint32_t* count;
   float *pre_v, *post_v;
   int32_t *pre_s, *post_s;
-  float *ptab, *mtab;
-  int ptab_n, mtab_n;
+  float *ptab, *mtab, *ytab;
+  int ptab_n, mtab_n, ytab_n;
   uint32_t pthr[64];
   int npthr;
   /* precomputed float constants (host double -> float, as the device does) */
@@ -708,10 +708,11 @@ mir_net* mir_create(const nsx_config* cfg) {
   if (cfg->drive == NSX_DRIVE_POISSON)
     r->npthr = orc_poisson_table((double)cfg->drive_inputs * cfg->drive_rate_hz * dt / 1000.0,
                                  r->pthr, 64);
-  if (cfg->plasticity == NSX_PLAST_PAIR) {
+  if (cfg->plasticity != NSX_PLAST_NONE) {
     r->ptab = trace_table(dt, cfg->tau_plus_ms, &r->ptab_n);
     r->mtab = trace_table(dt, cfg->tau_minus_ms, &r->mtab_n);
   }
+  if (cfg->plasticity == NSX_PLAST_TRIPLET) r->ytab = trace_table(dt, cfg->tau_y_ms, &r->ytab_n);
   nsx_state_view none;
   memset(&none, 0, sizeof(none));
   for (uint64_t i = 0; i < n; ++i) {
@@ -740,7 +741,7 @@ void mir_destroy(mir_net* r) {
   free(r->v); free(r->u); free(r->gm); free(r->gh); free(r->gn); free(r->s0); free(r->s1);
   free(r->s2); free(r->drive); free(r->c); free(r->d); free(r->pre_v); free(r->post_v);
   free(r->refr); free(r->last); free(r->pre_s); free(r->post_s); free(r->count);
-  free(r->ptab); free(r->mtab);
+  free(r->ptab); free(r->mtab); free(r->ytab);
   free(r);
 }
 
@@ -786,7 +787,7 @@ int mir_build_csr(mir_net* r, const uint32_t* src, const uint32_t* dst, const fl
   }
   free(perm);
   r->m = m;
-  if (r->cfg.plasticity == NSX_PLAST_PAIR) build_csc(r);
+  if (r->cfg.plasticity != NSX_PLAST_NONE) build_csc(r);
   return 0;
 }
 
@@ -985,6 +986,7 @@ uint64_t mir_run(mir_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* sp_i
   uint64_t ne = cfg->n_exc;
   uint32_t* spk = (uint32_t*)malloc(n * 4 + 4);
   const int pair = cfg->plasticity == NSX_PLAST_PAIR;
+  const int trip = cfg->plasticity == NSX_PLAST_TRIPLET;
   for (uint32_t step = 0; step < n_steps; ++step, ++r->t) {
     uint64_t t = r->t;
     int32_t ti = (int32_t)t;
@@ -1078,6 +1080,7 @@ uint64_t mir_run(mir_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* sp_i
       }
       if (spiked) {
         r->count[i] += 1;
+        if (trip && i < ne) r->post_s[i] = r->last[i]; /* previous post spike */
         r->last[i] = ti;
         spk[ns++] = (uint32_t)i;
         if (count < cap) { sp_step[count] = (uint32_t)t; sp_id[count] = (uint32_t)i; }
@@ -1102,6 +1105,13 @@ uint64_t mir_run(mir_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* sp_i
         float wk = r->w[k];
         int64_t q = (int64_t)llrint((double)wk * ldexp(1.0, r->S));
         r->ring[((size_t)sl * ch + chn) * n + j] += q;
+        if (trip && i < ne && j < ne) { /* pre spike: depression by the post's last spike */
+          float x = tab(r->mtab, r->mtab_n, (int64_t)ti - r->last[j]);
+          float dlt = cfg->a_minus * x;
+          float wn = wk - dlt;
+          wn = wn < cfg->w_min ? cfg->w_min : (wn > cfg->w_max ? cfg->w_max : wn);
+          r->w[k] = wn;
+        }
         if (pair && i < ne && j < ne) {
           float p = tab(r->mtab, r->mtab_n, (int64_t)ti - r->post_s[j]);
           float x = r->post_v[j] * p;
@@ -1132,6 +1142,27 @@ uint64_t mir_run(mir_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* sp_i
         x = x * p;
         r->post_v[j] = x + 1.0f;
         r->post_s[j] = ti;
+      }
+    }
+    /* ---- triplet: potentiation on E->E in-synapses of spiking excitatory targets ---- */
+    if (trip) {
+      for (uint64_t s = 0; s < ns; ++s) {
+        uint32_t j = spk[s];
+        if (j >= ne) continue;
+        for (uint64_t e = r->csc_ptr[j]; e < r->csc_ptr[j + 1]; ++e) {
+          uint32_t i = r->csc_src[e];
+          uint64_t k = r->csc_slot[e];
+          int32_t l = r->last[i];
+          if (l == ti || l == NEVER_STEP) continue; /* same step: the pre spike's LTD only */
+          float x1 = tab(r->ptab, r->ptab_n, (int64_t)ti - l);
+          float x2 = tab(r->ytab, r->ytab_n, (int64_t)ti - r->post_s[j]);
+          float dw = cfg->a_plus * x1;
+          float y = cfg->a3_plus * x2;
+          dw = dw + y;
+          float wn = r->w[k] + dw;
+          wn = wn < cfg->w_min ? cfg->w_min : (wn > cfg->w_max ? cfg->w_max : wn);
+          r->w[k] = wn;
+        }
       }
     }
   }

@@ -123,6 +123,17 @@ def test_izhikevich_stdp_bitexact(nsx, orc):
         "STDP should have changed weights"
 
 
+def test_izhikevich_triplet_bitexact(nsx, orc):
+    spec = nsx.configs.izh_stdp_network(n=4000, seed=23, record_raster=True,
+                                        raster_capacity=1 << 22, rule="triplet")
+    fields = ["v", "u", "spike_count", "last_spike_step", "post_step"]
+    st, g, m = _run_pair(nsx, orc, spec, 300, fields)
+    assert st.spikes > 1000
+    _assert_same(st, g, m)
+    assert not np.array_equal(_bits(g[3]), _bits(orc.MirrorNet.from_spec(spec).weights())), \
+        "STDP should have changed weights"
+
+
 def test_izhikevich_graph_equals_direct(nsx, orc):
     spec = nsx.configs.izh_stdp_network(n=3000, seed=4, record_raster=True,
                                         raster_capacity=1 << 22)

@@ -172,6 +172,39 @@ def bench_population(ns, steps=2000, tune_steps=50):
             "fitness_mean": float(fit.mean()), "fitness_best": float(fit.max())}
 
 
+def bench_large(ns, n=100_000_000, steps=300, drive_scale=1.0):
+    """10^8 LIF neurons, 10^10 synapses, pair STDP on E->E: device build, then steps.  At the
+    workload's drive the network is silent and the neuron sweep (HBM-bound) is the whole step;
+    drive_scale multiplies the Poisson weight to run the same network active."""
+    from neuralsim_amd import configs
+    spec = configs.lif_large_network(n=n, drive_scale=drive_scale)
+    t0 = time.perf_counter()
+    with ns.SimulationEngine.from_spec(spec) as eng:
+        build_s = time.perf_counter() - t0
+        eng.set_graph_steps(50)
+        eng.step(50)
+        t0 = time.perf_counter()
+        st = eng.step(steps)
+        wall = time.perf_counter() - t0
+        eng.set_graph_steps(0)
+        eng.set_profiling(True)
+        pst = eng.step(50)
+        eng.set_profiling(False)
+        ms_u, n_u, _ = eng.kernel_time(0)
+        ms_s, n_s, _ = eng.kernel_time(1)
+    upd_us = ms_u / max(n_u, 1) * 1e3
+    upd_b = neuron_bytes(spec.cfg, spec.n, pst.spikes / 50.0)
+    sim_s = steps * spec.cfg.dt_ms / 1e3
+    return {"neurons": spec.n, "synapses": spec.m, "drive_scale": drive_scale,
+            "build_s": build_s, "steps": steps,
+            "ms_per_step": wall * 1e3 / steps, "x_realtime": sim_s / wall,
+            "syn_events_per_s": st.events / wall, "spikes": int(st.spikes),
+            "k_update": {"avg_us": upd_us, "alg_bytes_per_launch": upd_b,
+                         "achieved_GBs": upd_b / (upd_us * 1e-6) / 1e9,
+                         "frac_of_hbm_peak": upd_b / (upd_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
+            "k_synapses_avg_us": ms_s / max(n_s, 1) * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +220,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hh", action="store_true", help="skip the HH surrogate comparison")
     ap.add_argument("--no-pop", action="store_true", help="skip the batched population")
+    ap.add_argument("--no-large", action="store_true", help="skip the 10^8-neuron network")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -293,6 +327,15 @@ def main():
         log("bench: batched population (256 networks)")
         out["population"] = bench_population(ns)
         log(f"bench: population {out['population']['ms_per_step']:.3f} ms/step")
+    if not args.no_large:
+        log("bench: 10^8 LIF neurons, 10^10 synapses")
+        for key, scale, steps in [("large_scale", 1.0, 300), ("large_scale_active", 60.0, 100)]:
+            try:
+                out[key] = bench_large(ns, steps=steps, drive_scale=scale)
+                log(f"bench: {key} {out[key]['ms_per_step']:.3f} ms/step")
+            except ns.NsxError as e:  # recorded, not fatal: the headline line still prints
+                out[key] = {"error": str(e)}
+                log(f"bench: {key} failed: {e}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, events / args.steps, args.cpu_steps)
     print(json.dumps(out), flush=True)

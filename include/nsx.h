@@ -228,7 +228,10 @@ nsx_status nsx_read_raster(nsx_ctx* ctx, uint32_t* steps, uint32_t* neurons, uin
 nsx_status nsx_read_step_counts(nsx_ctx* ctx, uint32_t* counts, uint64_t cap, uint64_t* n_out);
 
 /* Kernel timing: when enabled, every launch of the named kernel class in nsx_step is bracketed
- * by hipEvents on the stream it runs on.  kind: 0 neuron update, 1 propagation, 2 STDP-LTD. */
+ * by hipEvents on the stream it runs on (steps run as direct launches, not graphs).  Enabling
+ * starts a fresh measurement; disabling keeps it readable.  kind: 0 neuron update, 1 synapse
+ * pass (propagation and STDP); bytes report the STDP slots of the last call (1: at pre spikes,
+ * 2: at post spikes). */
 nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable);
 nsx_status nsx_kernel_time(nsx_ctx* ctx, int kind, double* total_ms, uint64_t* launches,
                            uint64_t* bytes);

@@ -243,3 +243,17 @@ def population_single(base, net):
     c.n_exc = net.n_exc
     c.seed = net.seed
     return c
+
+
+def lif_large_network(n=100_000_000, seed=5, steps=10_000, plasticity=True, drive_scale=1.0):
+    """Single-GPU large scale: the plain LIF network of lif_network (80/20, in-degree 100,
+    delays 1-5 ms, Poisson 50 x 10 Hz x 0.3 mV, dt 0.1 ms) at 10^8 neurons / 10^10 synapses,
+    with pair STDP on E->E (StdpRule defaults) and per-neuron traces."""
+    spec = lif_network(n=n, seed=seed, drive_scale=drive_scale, steps=steps,
+                       plasticity=plasticity)
+    c = spec.cfg
+    c.a_plus, c.a_minus, c.tau_plus_ms, c.tau_minus_ms = 0.01, 0.012, 20.0, 20.0
+    c.w_min, c.w_max = 0.0, 10.0
+    spec.name = f"lif_stdp_{n}"
+    spec.description = "LIF 80/20, in-degree 100, delays 1-5 ms, dt 0.1 ms, pair STDP E->E"
+    return spec

@@ -660,11 +660,13 @@ nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph) {
 
 nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  // switching on starts a fresh measurement; switching off keeps it for nsx_kernel_time
+  if (enable && !ctx->profiling)
+    for (int k = 0; k < 3; ++k) {
+      ctx->kt_ms[k] = 0;
+      ctx->kt_launches[k] = 0;
+    }
   ctx->profiling = enable != 0;
-  for (int k = 0; k < 3; ++k) {
-    ctx->kt_ms[k] = 0;
-    ctx->kt_launches[k] = 0;
-  }
   return NSX_OK;
 }
 

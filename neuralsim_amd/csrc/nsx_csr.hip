@@ -19,38 +19,43 @@
 
 namespace nsx {
 
+// Element-wise kernels stride over their domain: a dispatch counts its work-items in 32 bits,
+// so a launch is capped (grid_for) and every thread takes every (grid size)-th element.
+#define GRID_STRIDE(k, count) \
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < (count); \
+       k += (uint64_t)gridDim.x * blockDim.x)
+
 __global__ void k_check_list(const uint32_t* src, const uint32_t* dst, const uint8_t* delay,
                              uint64_t m, uint32_t n, uint32_t D, unsigned int* bad) {
-  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  if (src[k] >= n || dst[k] >= n || delay[k] < 1u || delay[k] >= D) atomicOr(bad, 1u);
+  GRID_STRIDE(k, m) {
+    if (src[k] >= n || dst[k] >= n || delay[k] < 1u || delay[k] >= D) atomicOr(bad, 1u);
+  }
 }
 
 __global__ void k_iota(uint32_t* v, uint64_t m) {
-  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < m) v[k] = (uint32_t)k;
+  GRID_STRIDE(k, m) v[k] = (uint32_t)k;
 }
 
 // row_ptr[r] = first sorted position whose key >= r (keys sorted ascending)
 __global__ void k_row_ptr(const uint32_t* keys, uint64_t m, uint32_t n, uint64_t* row_ptr) {
-  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > m) return;
-  uint32_t lo = (k == 0) ? 0u : keys[k - 1] + 1u;
-  uint32_t hi = (k == m) ? n : keys[k];
-  for (uint32_t r = lo; r <= hi && r <= n; ++r) row_ptr[r] = k;
-  if (k == m && m == 0)
-    for (uint32_t r = 0; r <= n; ++r) row_ptr[r] = 0;
+  GRID_STRIDE(k, m + 1) {
+    uint32_t lo = (k == 0) ? 0u : keys[k - 1] + 1u;
+    uint32_t hi = (k == m) ? n : keys[k];
+    for (uint32_t r = lo; r <= hi && r <= n; ++r) row_ptr[r] = k;
+    if (k == m && m == 0)
+      for (uint32_t r = 0; r <= n; ++r) row_ptr[r] = 0;
+  }
 }
 
 __global__ void k_gather(const uint32_t* perm, const uint32_t* dst_in, const float* w_in,
                          const uint8_t* d_in, uint64_t m, uint32_t* col, float* w,
                          uint8_t* delay) {
-  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  uint32_t q = perm[k];
-  col[k] = dst_in[q];
-  w[k] = w_in[q];
-  delay[k] = d_in[q];
+  GRID_STRIDE(k, m) {
+    uint32_t q = perm[k];
+    col[k] = dst_in[q];
+    w[k] = w_in[q];
+    delay[k] = d_in[q];
+  }
 }
 
 // Fixed in-degree generator: synapse (j, k) from Philox(seed; j, k, CONN).
@@ -58,54 +63,178 @@ __global__ void k_gen_fixed_indegree(uint64_t seed, uint32_t n, uint32_t n_exc, 
                                      float w_exc_lo, float w_exc_hi, float w_inh_lo,
                                      float w_inh_hi, uint32_t d_lo, uint32_t d_span,
                                      uint32_t* src, uint32_t* dst, float* w, uint8_t* delay) {
-  uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t m = (uint64_t)n * k_in;
-  if (idx >= m) return;
-  uint32_t j = (uint32_t)(idx / k_in), k = (uint32_t)(idx % k_in);
-  gen_synapse(seed, n, n_exc, w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_span, j, k,
-              src[idx], w[idx], delay[idx]);
-  dst[idx] = j;
+  GRID_STRIDE(idx, (uint64_t)n * k_in) {
+    uint32_t j = (uint32_t)(idx / k_in), k = (uint32_t)(idx % k_in);
+    gen_synapse(seed, n, n_exc, w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_span, j, k,
+                src[idx], w[idx], delay[idx]);
+    dst[idx] = j;
+  }
+}
+
+// ---- direct fixed in-degree build (no global sort; any number of synapses) ------------------
+struct ToU64 {
+  __host__ __device__ unsigned long long operator()(uint32_t x) const { return x; }
+};
+
+// Synapse (j, k) of the generator is row entry number (j * k_in + k) in insertion order; CSR by
+// source in that order is what a stable sort by source gives (Network::finalize,
+// network/mod.rs:79-106).  Pass 1 counts out-degrees, pass 2 drops every synapse into its row
+// at an atomic cursor together with k, and a per-row sort by (target, k) restores insertion
+// order.
+
+__global__ void k_gen_count(uint64_t seed, uint32_t n, uint32_t n_exc, uint32_t k_in,
+                            uint32_t* cnt) {
+  GRID_STRIDE(idx, (uint64_t)n * k_in) {
+    const uint32_t j = (uint32_t)(idx / k_in), k = (uint32_t)(idx % k_in);
+    U4 r = draw(seed, j, k, kStreamConn);
+    uint32_t src = (uint32_t)(((uint64_t)r.x * (uint64_t)(n - 1)) >> 32);
+    if (src >= j) src += 1;
+    atomicAdd(&cnt[src], 1u);
+  }
+}
+
+__global__ void k_gen_fill(uint64_t seed, uint32_t n, uint32_t n_exc, uint32_t k_in,
+                           float w_exc_lo, float w_exc_hi, float w_inh_lo, float w_inh_hi,
+                           uint32_t d_lo, uint32_t d_span, const uint64_t* row_ptr,
+                           uint32_t* cursor, uint32_t* col, float* w, uint8_t* delay,
+                           uint8_t* kk) {
+  GRID_STRIDE(idx, (uint64_t)n * k_in) {
+    const uint32_t j = (uint32_t)(idx / k_in), k = (uint32_t)(idx % k_in);
+    uint32_t src;
+    float wv;
+    uint8_t dv;
+    gen_synapse(seed, n, n_exc, w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_span, j, k,
+                src, wv, dv);
+    const uint64_t pos = row_ptr[src] + atomicAdd(&cursor[src], 1u);
+    col[pos] = j;
+    w[pos] = wv;
+    delay[pos] = dv;
+    kk[pos] = (uint8_t)k;
+  }
+}
+
+// Every row filled exactly: the cursors equal the row lengths.
+__global__ void k_check_fill(const uint64_t* row_ptr, const uint32_t* cursor, uint32_t n,
+                             unsigned int* bad) {
+  GRID_STRIDE(i, n) {
+    if ((uint64_t)cursor[i] != row_ptr[i + 1] - row_ptr[i]) atomicOr(bad, 1u);
+  }
+}
+
+// One wavefront per row (rows of at most 256 entries): bitonic sort of (col, k) in LDS, then the
+// row's col / w / delay rewritten in that order.
+__global__ __launch_bounds__(256) void k_row_sort(const uint64_t* row_ptr, uint32_t n,
+                                                  uint32_t* col, float* w, uint8_t* delay,
+                                                  const uint8_t* kk) {
+  __shared__ unsigned long long key[4][256];
+  __shared__ uint32_t scol[4][256];
+  __shared__ float sw[4][256];
+  __shared__ uint8_t sd[4][256];
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  for (uint64_t row = (uint64_t)blockIdx.x * 4 + wid; row < n;
+       row += (uint64_t)gridDim.x * 4) {
+    const uint64_t beg = row_ptr[row];
+    const uint32_t len = (uint32_t)(row_ptr[row + 1] - beg);
+    if (len < 2) continue;  // wave-uniform
+    uint32_t P = 2;
+    while (P < len) P <<= 1;
+    for (uint32_t e = lane; e < P; e += 64) {
+      if (e < len) {
+        const uint32_t c = col[beg + e];
+        scol[wid][e] = c;
+        sw[wid][e] = w[beg + e];
+        sd[wid][e] = delay[beg + e];
+        // key: (target, k) in the high 40 bits, the entry's position in the low 8 bits
+        key[wid][e] = ((unsigned long long)c << 24) | ((unsigned long long)kk[beg + e] << 8) | e;
+      } else {
+        key[wid][e] = ~0ull;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t e = lane; e < P; e += 64) {
+          const uint32_t o = e ^ stride;
+          if (o > e) {
+            const bool up = (e & size) == 0;
+            const unsigned long long a = key[wid][e], b = key[wid][o];
+            if ((a > b) == up) {
+              key[wid][e] = b;
+              key[wid][o] = a;
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+    for (uint32_t e = lane; e < len; e += 64) {
+      const uint32_t from = (uint32_t)(key[wid][e] & 0xffu);
+      col[beg + e] = scol[wid][from];
+      w[beg + e] = sw[wid][from];
+      delay[beg + e] = sd[wid][from];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // LDS is reused by the next row
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 }
 
 __global__ void k_check_csr(const uint64_t* row_ptr, const uint32_t* col, const uint8_t* delay,
                             uint32_t n, uint64_t m, uint32_t D, unsigned int* bad) {
-  uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) {
-    if (row_ptr[k] > row_ptr[k + 1]) atomicOr(bad, 1u);
-  }
-  if (k == 0 && (row_ptr[0] != 0 || row_ptr[n] != m)) atomicOr(bad, 2u);
-  if (k < m) {
-    if (col[k] >= n) atomicOr(bad, 4u);
-    if (delay && (delay[k] < 1u || delay[k] >= D)) atomicOr(bad, 8u);
+  GRID_STRIDE(k, (m > n ? m : (uint64_t)n) + 1) {
+    if (k < n) {
+      if (row_ptr[k] > row_ptr[k + 1]) atomicOr(bad, 1u);
+    }
+    if (k == 0 && (row_ptr[0] != 0 || row_ptr[n] != m)) atomicOr(bad, 2u);
+    if (k < m) {
+      if (col[k] >= n) atomicOr(bad, 4u);
+      if (delay && (delay[k] < 1u || delay[k] >= D)) atomicOr(bad, 8u);
+    }
   }
 }
 
 // E->E transpose: count per excitatory target, then scatter (slot, source).
 __global__ void k_csc_count(const uint64_t* row_ptr, const uint32_t* col, uint32_t n_exc,
                             unsigned long long* cnt) {
-  uint32_t i = blockIdx.x;  // one block per excitatory source row
-  if (i >= n_exc) return;
-  for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
-    uint32_t j = col[k];
-    if (j < n_exc) atomicAdd(&cnt[j], 1ull);
-  }
-}
-
-__global__ void k_csc_fill(const uint64_t* row_ptr, const uint32_t* col, uint32_t n_exc,
-                           unsigned long long* cursor, uint32_t* slot, uint32_t* srcs) {
-  uint32_t i = blockIdx.x;
-  if (i >= n_exc) return;
-  for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
-    uint32_t j = col[k];
-    if (j < n_exc) {
-      unsigned long long p = atomicAdd(&cursor[j], 1ull);
-      slot[p] = (uint32_t)k;
-      srcs[p] = i;
+  for (uint32_t i = blockIdx.x; i < n_exc; i += gridDim.x) {  // a block per source row
+    for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
+      uint32_t j = col[k];
+      if (j < n_exc) atomicAdd(&cnt[j], 1ull);
     }
   }
 }
 
-static inline uint32_t blocks_for(uint64_t m, uint32_t b) { return (uint32_t)((m + b - 1) / b); }
+__global__ void k_csc_fill(const uint64_t* row_ptr, const uint32_t* col, uint32_t n_exc,
+                           unsigned long long* cursor, uint64_t* slot, uint32_t* srcs) {
+  for (uint32_t i = blockIdx.x; i < n_exc; i += gridDim.x) {
+    for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
+      uint32_t j = col[k];
+      if (j < n_exc) {
+        unsigned long long p = atomicAdd(&cursor[j], 1ull);
+        slot[p] = k;
+        srcs[p] = i;
+      }
+    }
+  }
+}
+
+// Every target's in-slots filled: cursor[j] reached csc_ptr[j + 1].
+__global__ void k_check_csc(const uint64_t* csc_ptr, const unsigned long long* cursor,
+                            uint32_t n_exc, unsigned int* bad) {
+  GRID_STRIDE(j, n_exc) {
+    if (cursor[j] != csc_ptr[j + 1]) atomicOr(bad, 1u);
+  }
+}
+
+// Blocks for `m` items at `b` per block, capped: the kernels stride over the rest.
+static inline uint32_t blocks_for(uint64_t m, uint32_t b) {
+  const uint64_t g = (m + b - 1) / b;
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(g, 1), 1u << 16);
+}
 
 static nsx_status free_network(nsx_ctx* ctx) {
   dev_free(ctx, ctx->row_ptr);
@@ -155,7 +284,8 @@ nsx_status build_csc_device(nsx_ctx* ctx) {
   if (ne == 0) return NSX_OK;
   unsigned long long* cnt = nullptr;
   if ((s = dev_alloc(ctx, (void**)&cnt, (size_t)ne * 8, true))) return s;
-  hipLaunchKernelGGL(k_csc_count, dim3(ne), dim3(128), 0, ctx->stream, ctx->row_ptr, ctx->col,
+  hipLaunchKernelGGL(k_csc_count, dim3(blocks_for(ne, 1)), dim3(128), 0, ctx->stream,
+                     ctx->row_ptr, ctx->col,
                      ne, cnt);
   NSX_HIP(ctx, hipGetLastError());
   // exclusive scan into csc_ptr[1..]
@@ -172,15 +302,24 @@ nsx_status build_csc_device(nsx_ctx* ctx) {
   NSX_HIP(ctx, hipMemcpyAsync(&tot, ctx->csc_ptr + ne, 8, hipMemcpyDeviceToHost, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   dev_free(ctx, tmp);
-  if ((s = dev_alloc(ctx, (void**)&ctx->csc_slot, tot * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->csc_slot, tot * 8, false))) return s;
   if ((s = dev_alloc(ctx, (void**)&ctx->csc_src, tot * 4, false))) return s;
   NSX_HIP(ctx, hipMemcpyAsync(cnt, ctx->csc_ptr, (size_t)ne * 8, hipMemcpyDeviceToDevice,
                               ctx->stream));
-  hipLaunchKernelGGL(k_csc_fill, dim3(ne), dim3(128), 0, ctx->stream, ctx->row_ptr, ctx->col, ne,
-                     cnt, ctx->csc_slot, ctx->csc_src);
+  hipLaunchKernelGGL(k_csc_fill, dim3(blocks_for(ne, 1)), dim3(128), 0, ctx->stream,
+                     ctx->row_ptr, ctx->col, ne, cnt, ctx->csc_slot, ctx->csc_src);
   NSX_HIP(ctx, hipGetLastError());
+  unsigned int* bad = nullptr;
+  if ((s = dev_alloc(ctx, (void**)&bad, 4, true))) return s;
+  hipLaunchKernelGGL(k_check_csc, dim3(blocks_for(ne, 256)), dim3(256), 0, ctx->stream,
+                     ctx->csc_ptr, cnt, ne, bad);
+  NSX_HIP(ctx, hipGetLastError());
+  unsigned int hb = 0;
+  NSX_HIP(ctx, hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, bad);
   dev_free(ctx, cnt);
+  if (hb) return set_error(ctx, NSX_E_HIP, "transpose index incomplete");
   ctx->csc_m = tot;
   return NSX_OK;
 }
@@ -306,24 +445,89 @@ nsx_status nsx_generate_fixed_indegree(nsx_ctx* ctx, uint32_t k_in, float w_exc_
   if (ctx->n < 2) return set_error(ctx, NSX_E_INVALID, "generator needs n_neurons >= 2");
   if (d_lo < 1 || d_hi < d_lo || d_hi >= (uint32_t)ctx->D)
     return set_error(ctx, NSX_E_INVALID, "delays must satisfy 1 <= d_lo <= d_hi <= ring_slots-1");
-  const uint64_t m = ctx->n * (uint64_t)k_in;
-  uint32_t *ds = nullptr, *dd = nullptr;
-  float* dw = nullptr;
-  uint8_t* dl = nullptr;
+  if (k_in > 256) return set_error(ctx, NSX_E_INVALID, "k_in must be <= 256");
+  const uint32_t n = (uint32_t)ctx->n;
+  const uint64_t m = (uint64_t)n * k_in;
+  const uint32_t d_span = d_hi - d_lo + 1;
   nsx_status s;
-  if ((s = dev_alloc(ctx, (void**)&ds, m * 4, false)) ||
-      (s = dev_alloc(ctx, (void**)&dd, m * 4, false)) ||
-      (s = dev_alloc(ctx, (void**)&dw, m * 4, false)) || (s = dev_alloc(ctx, (void**)&dl, m, false)))
-    return s;
+  free_network(ctx);
+  uint32_t *cnt = nullptr, *cur = nullptr;
+  uint8_t* kk = nullptr;
+  if ((s = dev_alloc(ctx, (void**)&cnt, (size_t)n * 4, true))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->row_ptr, ((size_t)n + 1) * 8, true))) return s;
   if (m) {
-    hipLaunchKernelGGL(k_gen_fixed_indegree, dim3(blocks_for(m, 256)), dim3(256), 0, ctx->stream,
-                       ctx->cfg.seed, (uint32_t)ctx->n, (uint32_t)ctx->cfg.n_exc, k_in, w_exc_lo,
-                       w_exc_hi, w_inh_lo, w_inh_hi, d_lo, d_hi - d_lo + 1, ds, dd, dw, dl);
+    hipLaunchKernelGGL(k_gen_count, dim3(blocks_for(m, 256)), dim3(256), 0, ctx->stream,
+                       ctx->cfg.seed, n, (uint32_t)ctx->cfg.n_exc, k_in, cnt);
     NSX_HIP(ctx, hipGetLastError());
   }
-  s = build_csr_device(ctx, ds, dd, dw, dl, m);
-  dev_free(ctx, ds); dev_free(ctx, dd); dev_free(ctx, dw); dev_free(ctx, dl);
-  return s;
+  // row_ptr[1..n] = inclusive sum of the out-degrees (64-bit accumulation); the longest row
+  // must fit the row sort
+  size_t tmp_bytes = 0, tmp2 = 0;
+  uint32_t* dmax = nullptr;
+  if ((s = dev_alloc(ctx, (void**)&dmax, 4, true))) return s;
+  hipcub::TransformInputIterator<unsigned long long, ToU64, const uint32_t*> wide(cnt, ToU64());
+  NSX_HIP(ctx, hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, wide,
+                                                 (unsigned long long*)ctx->row_ptr + 1, (int)n,
+                                                 ctx->stream));
+  NSX_HIP(ctx, hipcub::DeviceReduce::Max(nullptr, tmp2, cnt, dmax, (int)n, ctx->stream));
+  void* tmp = nullptr;
+  if ((s = dev_alloc(ctx, &tmp, std::max(tmp_bytes, tmp2), false))) return s;
+  NSX_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, wide,
+                                                 (unsigned long long*)ctx->row_ptr + 1, (int)n,
+                                                 ctx->stream));
+  NSX_HIP(ctx, hipcub::DeviceReduce::Max(tmp, tmp2, cnt, dmax, (int)n, ctx->stream));
+  uint32_t maxdeg = 0;
+  NSX_HIP(ctx, hipMemcpyAsync(&maxdeg, dmax, 4, hipMemcpyDeviceToHost, ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, tmp);
+  dev_free(ctx, dmax);
+  if (maxdeg > 256) {
+    dev_free(ctx, cnt);
+    free_network(ctx);
+    return set_error(ctx, NSX_E_INVALID, "an out-degree above 256 (rows are sorted in LDS)");
+  }
+  cur = cnt;  // the counts become cursors
+  NSX_HIP(ctx, hipMemsetAsync(cur, 0, (size_t)n * 4, ctx->stream));
+  if ((s = dev_alloc(ctx, (void**)&ctx->col, m * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->w, m * 4, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&ctx->delay, m, false))) return s;
+  if ((s = dev_alloc(ctx, (void**)&kk, m, false))) return s;
+  if (m) {
+    hipLaunchKernelGGL(k_gen_fill, dim3(blocks_for(m, 256)), dim3(256), 0, ctx->stream,
+                       ctx->cfg.seed, n, (uint32_t)ctx->cfg.n_exc, k_in, w_exc_lo, w_exc_hi,
+                       w_inh_lo, w_inh_hi, d_lo, d_span, ctx->row_ptr, cur, ctx->col, ctx->w,
+                       ctx->delay, kk);
+    NSX_HIP(ctx, hipGetLastError());
+    unsigned int* bad = nullptr;
+    if ((s = dev_alloc(ctx, (void**)&bad, 4, true))) return s;
+    hipLaunchKernelGGL(k_check_fill, dim3(blocks_for(n, 256)), dim3(256), 0, ctx->stream,
+                       ctx->row_ptr, cur, n, bad);
+    NSX_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(n, 4)), dim3(256), 0, ctx->stream,
+                       ctx->row_ptr, n, ctx->col, ctx->w, ctx->delay, kk);
+    NSX_HIP(ctx, hipGetLastError());
+    unsigned int hb = 0;
+    NSX_HIP(ctx, hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
+    NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    dev_free(ctx, bad);
+    if (hb) {
+      dev_free(ctx, kk);
+      dev_free(ctx, cnt);
+      free_network(ctx);
+      return set_error(ctx, NSX_E_HIP, "generator rows not filled exactly");
+    }
+  }
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, kk);
+  dev_free(ctx, cnt);
+  ctx->m = m;
+  if ((s = validate_csr_device(ctx))) {
+    free_network(ctx);
+    return s;
+  }
+  if (ctx->cfg.plasticity != NSX_PLAST_NONE && (s = build_csc_device(ctx))) return s;
+  publish_network(ctx);
+  return NSX_OK;
 }
 
 nsx_status nsx_upload_csr(nsx_ctx* ctx, const uint64_t* row_ptr, const uint32_t* col,

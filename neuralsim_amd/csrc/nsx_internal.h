@@ -110,7 +110,7 @@ struct DevPtrs {
   float* w;
   const uint8_t* delay;
   const uint64_t* csc_ptr;
-  const uint32_t* csc_slot;
+  const uint64_t* csc_slot;
   const uint32_t* csc_src;
   // batched population
   const uint32_t* blk_net;  // network of each neuron block
@@ -178,7 +178,7 @@ struct nsx_ctx {
   float* w = nullptr;
   uint8_t* delay = nullptr;
   uint64_t* csc_ptr = nullptr;
-  uint32_t* csc_slot = nullptr;
+  uint64_t* csc_slot = nullptr;
   uint32_t* csc_src = nullptr;
   uint64_t csc_m = 0;
   std::vector<float> ptab_h, mtab_h, ytab_h;  // host copies of the trace decay tables

@@ -20,6 +20,8 @@
 //                atomic-throughput-bound propagation overlaps the latency-bound depression.
 // Kernels read the step index as *tbase + off so that a captured hipGraph of G steps can be
 // replayed (k_advance moves tbase by G at the end of the graph).
+#include <algorithm>
+
 #include "nsx_device.h"
 #include "nsx_hh.h"
 #include "nsx_internal.h"
@@ -166,9 +168,8 @@ __device__ inline uint64_t chunk_slot(const RowChunk& rc, uint32_t f, uint32_t& 
 // triplet: depression by the target's last spike.
 template <int PL>
 __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
-                               uint32_t b, uint32_t q, uint32_t lane) {
+                               uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane) {
   const uint32_t P = p.prop_split;
-  const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
   if (cnt <= q) return;
   const uint32_t mine = (cnt - q + P - 1) / P;  // rows q, q+P, ... of the segment
   const uint32_t tm = (uint32_t)t % p.D;
@@ -283,9 +284,8 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
 // trace; triplet: potentiation by the source's last spike and the target's previous spike.
 template <int PL>
 __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
-                             uint32_t b, uint32_t q, uint32_t lane) {
+                             uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane) {
   const uint32_t P = p.prop_split;
-  const uint32_t cnt = __builtin_amdgcn_readfirstlane(s.bcnt[b]);
   if (cnt <= q) return;
   const uint32_t mine = (cnt - q + P - 1) / P;
   unsigned long long slots = 0;
@@ -309,7 +309,7 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
       uint32_t r1 = r;
       const uint64_t e1 = two ? chunk_slot(rc, f + 64, r1) : e0;
       const uint32_t i0 = s.csc_src[e0], i1 = s.csc_src[e1];
-      const uint32_t k0 = s.csc_slot[e0], k1 = s.csc_slot[e1];
+      const uint64_t k0 = s.csc_slot[e0], k1 = s.csc_slot[e1];
       if (PL == NSX_PLAST_TRIPLET) {
         const int32_t l0 = s.last[i0], l1 = s.last[i1];
         // the source's propagation item owns same-step pairs; a source that never fired
@@ -372,8 +372,25 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
   }
 }
 
-// Work items: propagation (nblk * P) and depression (nblk_e * P), interleaved so that both
-// kinds are resident on every CU.
+// Work items: propagation (nblk * P) and post-spike STDP (nblk_e * P), interleaved so that both
+// kinds are resident on every CU.  The grid is capped (and fixed, for graph capture): each
+// wavefront owns items wave, wave + W, wave + 2W, ... (W wavefronts in the grid), looks up the
+// block spike counts of 64 of them at once (a lane each), and works only on those with spikes,
+// so the empty blocks of a sparse step cost one parallel load per 64 items.
+__device__ inline void decode_item(uint32_t wk, uint32_t both, uint32_t nprop, uint32_t nltd,
+                                   uint32_t P, bool& prop, uint32_t& b, uint32_t& q) {
+  uint32_t item;
+  if (wk < both) {
+    prop = (wk & 1u) == 0u;
+    item = wk >> 1;
+  } else {
+    prop = nprop > nltd;
+    item = wk - both / 2;
+  }
+  b = item / P;
+  q = item - b * P;
+}
+
 template <int PL>
 __global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, int32_t off) {
   __shared__ RowChunk sh[kBlock / 64];
@@ -386,21 +403,28 @@ __global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, in
   const uint32_t both = 2 * min(nprop, nltd);
   const uint32_t nwork = nprop + nltd;
   const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
-  for (uint32_t wk = wave; wk < nwork; wk += nwaves) {
-    bool prop;
-    uint32_t item;
-    if (wk < both) {
-      prop = (wk & 1u) == 0u;
-      item = wk >> 1;
-    } else {
-      prop = nprop > nltd;
-      item = wk - both / 2;
+  for (uint64_t base = wave; base < nwork; base += 64ull * nwaves) {
+    const uint64_t mine = base + (uint64_t)lane * nwaves;
+    uint32_t c = 0, qq = 0;
+    if (mine < nwork) {
+      bool pr;
+      uint32_t bb;
+      decode_item((uint32_t)mine, both, nprop, nltd, P, pr, bb, qq);
+      c = s.bcnt[bb];
     }
-    const uint32_t b = item / P, q = item - b * P;
-    if (prop)
-      propagate_item<PL>(p, s, sh[wid], t, b, q, lane);
-    else if (PL != NSX_PLAST_NONE)
-      depress_item<PL>(p, s, sh[wid], t, b, q, lane);
+    unsigned long long act = __ballot(mine < nwork && c > qq);
+    while (act) {
+      const uint32_t l = (uint32_t)(__ffsll((long long)act) - 1);
+      act &= act - 1ull;
+      bool prop;
+      uint32_t b, q;
+      decode_item((uint32_t)(base + (uint64_t)l * nwaves), both, nprop, nltd, P, prop, b, q);
+      const uint32_t cnt = __builtin_amdgcn_readfirstlane(__shfl(c, (int)l));
+      if (prop)
+        propagate_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
+      else if (PL != NSX_PLAST_NONE)
+        depress_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
+    }
   }
 }
 
@@ -434,12 +458,13 @@ UpdateFn pick_update(int model, int drive, int pl) {
   }
 }
 
-// one wavefront per work item
+// One wavefront per work item when every block spikes, capped: sparse steps leave most waves
+// idle and the cap bounds their cost.
 uint32_t syn_grid(const nsx_ctx* ctx) {
-  const bool pair = ctx->cfg.plasticity != NSX_PLAST_NONE;
-  const uint64_t nblk_e = (ctx->cfg.n_exc + kBlock - 1) / kBlock;
-  const uint64_t waves = ((uint64_t)ctx->p.nblk + (pair ? nblk_e : 0)) * ctx->p.prop_split;
-  return (uint32_t)((waves + kBlock / 64 - 1) / (kBlock / 64));
+  const bool plast = ctx->cfg.plasticity != NSX_PLAST_NONE;
+  const uint64_t waves = (uint64_t)ctx->p.nblk * (plast ? 2 : 1) * ctx->p.prop_split;
+  const uint64_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+  return (uint32_t)std::min<uint64_t>(blocks, 16384);
 }
 
 // Enqueue the launches of one step with offset `off` from *tbase.

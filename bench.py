@@ -172,7 +172,7 @@ def bench_population(ns, steps=2000, tune_steps=50):
             "fitness_mean": float(fit.mean()), "fitness_best": float(fit.max())}
 
 
-def bench_large(ns, n=100_000_000, steps=300, drive_scale=1.0):
+def bench_large(ns, n=100_000_000, steps=300, drive_scale=1.0, graph=50):
     """10^8 LIF neurons, 10^10 synapses, pair STDP on E->E: device build, then steps.  At the
     workload's drive the network is silent and the neuron sweep (HBM-bound) is the whole step;
     drive_scale multiplies the Poisson weight to run the same network active."""
@@ -181,7 +181,7 @@ def bench_large(ns, n=100_000_000, steps=300, drive_scale=1.0):
     t0 = time.perf_counter()
     with ns.SimulationEngine.from_spec(spec) as eng:
         build_s = time.perf_counter() - t0
-        eng.set_graph_steps(50)
+        eng.set_graph_steps(graph)
         eng.step(50)
         t0 = time.perf_counter()
         st = eng.step(steps)
@@ -331,7 +331,7 @@ def main():
         log("bench: 10^8 LIF neurons, 10^10 synapses")
         for key, scale, steps in [("large_scale", 1.0, 300), ("large_scale_active", 60.0, 100)]:
             try:
-                out[key] = bench_large(ns, steps=steps, drive_scale=scale)
+                out[key] = bench_large(ns, steps=steps, drive_scale=scale, graph=args.graph)
                 log(f"bench: {key} {out[key]['ms_per_step']:.3f} ms/step")
             except ns.NsxError as e:  # recorded, not fatal: the headline line still prints
                 out[key] = {"error": str(e)}

@@ -223,8 +223,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 10^8-neuron network")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU (asked for --gpus {args.gpus}); runs on several "
+              "GPUs are not supported", file=sys.stderr)
         sys.exit(2)
 
     import neuralsim_amd as ns

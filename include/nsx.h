@@ -7,7 +7,7 @@
  *
  *   1. nsx_backend_*  -- one-to-one replacements of GpuBackend's methods, same argument meaning,
  *      same per-call host round trip.  A Rust `extern "C"` block binds these where
- *      gpu_backend.rs is compiled today (binding shown in INTEGRATION.md).
+ *      gpu_backend.rs is compiled today.
  *   2. nsx_*          -- the device-resident engine: the network, neuron state, delay rings,
  *      plasticity traces and drives live in HBM; nsx_step() advances many steps per call and
  *      only statistics cross PCIe.  This is what SimulationEngine::step()/simulate_ms()

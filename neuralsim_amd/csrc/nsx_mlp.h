@@ -1,13 +1,16 @@
 // nsx_mlp.h -- the HH surrogate MLP 5 -> 64 -> 64 -> 4 (ReLU) with bf16 weights and FP32
 // accumulation, evaluated and differentiated by one wavefront per chunk of 64 samples.
 //
-// Layer 1 (K = 5) runs on the vector ALUs in FP32 with the bf16-rounded weights: the normalised
-// inputs stay exact, and a K = 5 contraction would waste 11/16 of an MFMA.  Layers 2 and 3 run on
-// gfx950 bf16 MFMA (v_mfma_f32_32x32x16_bf16 / v_mfma_f32_16x16x32_bf16) with activations
-// rounded to bf16 and staged through LDS.  The output layer's weights are carried as two bf16
-// words, W3 = hi + lo (lo = bf16(W3 - hi)), and multiplied in two MFMA passes: its rounding
-// error lands directly on the outputs, and a single bf16 word there is the largest error of the
-// whole surrogate step.  Operand layouts:
+// All three layers run on gfx950 bf16 MFMA (v_mfma_f32_32x32x16_bf16 / v_mfma_f32_16x16x32_bf16)
+// with activations rounded to bf16 and staged through LDS.  Layer 1 (K = 5) carries each
+// normalised input as two bf16 words, x = hi + lo (lo = bf16(x - hi), 16 significant bits), and
+// its bias as two words against constant-1 inputs: K = 5 + 5 + 2 <= 16, one MFMA per 32x32 tile,
+// so neither the inputs nor the bias are cut to 8 bits.  Layer 2's bias initialises the
+// accumulators.  The output layer's weights are carried as two bf16 words, W3 = hi + lo, and
+// multiplied in two MFMA passes: its rounding error lands directly on the outputs, and a single
+// bf16 word there is the largest error of the whole surrogate step.  ReLU runs on the packed
+// bf16 results as a signed 16-bit max against 0 (the sign bit of a bf16 is that of an int16, so
+// relu-then-round and round-then-max agree).  Operand layouts:
 //   32x32x16: lane l holds A[m = l%32][k = 8(l/32)+j], B[k = 8(l/32)+j][n = l%32] (j < 8);
 //             C register i holds (m = 8(i/4) + 4(l/32) + i%4, n = l%32)
 //   16x16x32: lane l holds A[m = l%16][k = 8(l/16)+j], B[k = 8(l/16)+j][n = l%16];
@@ -46,10 +49,14 @@ struct MlpNorm {
 };
 constexpr int kNormFloats = (int)(sizeof(MlpNorm) / 4);
 
+// Layer-1 columns (K = 16): the inputs' high words, their low words, two constant-1 columns
+// against the bias's two words, zero padding.
+constexpr int kL1Lo = kMlpIn, kL1Bias = 2 * kMlpIn;
+
 // Weights as the kernels read them (rebuilt from the FP32 masters after every optimiser step).
 struct __attribute__((aligned(16))) MlpImage {
-  float w1[kMlpH * kMlpIn];  // bf16-rounded, as FP32, [h][k]
-  float b1[kMlpH], b2[kMlpH], b3[kMlpOut];
+  float b2[kMlpH], b3[kMlpOut];  // FP32
+  bf16x8 w1f[2][64];   // A fragments of [W1 | W1 | b1 hi | b1 lo | 0] [h1][16]: m-tile of 32
   bf16x8 w2f[8][64];   // A fragments of W2 [h2][h1]: tile mt*4+ks (m-tile of 32, k-step of 16)
   bf16x8 w3f[2][64];   // A fragments of W3 [o][h2] (hi words), o padded to 16: k-step of 32
   bf16x8 w3lf[2][64];  // the same for the lo words
@@ -57,21 +64,20 @@ struct __attribute__((aligned(16))) MlpImage {
   bf16x8 w3tf[2][64];  // A fragments of W3^T [h2][o] (hi words), o padded to 16: m-tile of 32
   bf16x8 w3tlf[2][64]; // the same for the lo words
 };
-constexpr int kImageFrags = 24;  // fragment arrays of 64 lanes
-static_assert(offsetof(MlpImage, w2f) % 16 == 0, "fragment alignment");
+constexpr int kImageFrags = 26;  // fragment arrays of 64 lanes
+static_assert(offsetof(MlpImage, w1f) % 16 == 0, "fragment alignment");
 
 // The image's FP32 head and the normalisation, as staged into LDS by every block that evaluates
 // the MLP: read with wave-uniform addresses, they would otherwise be vector loads with a wait each
 // (the kernels' own global stores keep the compiler from proving them invariant for scalar loads).
-struct MlpScalars {
-  float w1[kMlpH * kMlpIn];
-  float b1[kMlpH], b2[kMlpH], b3[kMlpOut];
+struct __attribute__((aligned(16))) MlpScalars {
+  float b2[kMlpH], b3[kMlpOut];
   MlpNorm norm;
 };
-constexpr int kScalarHead = kMlpH * kMlpIn + 2 * kMlpH + kMlpOut;  // floats taken from the image
+constexpr int kScalarHead = kMlpH + kMlpOut;  // floats taken from the image
 constexpr int kScalarFloats = kScalarHead + kNormFloats;
-static_assert(offsetof(MlpImage, w2f) == kScalarHead * 4 && sizeof(MlpScalars) == kScalarFloats * 4,
-              "the image starts with w1, b1, b2, b3");
+static_assert(offsetof(MlpImage, w1f) == kScalarHead * 4 && sizeof(MlpScalars) == kScalarFloats * 4,
+              "the image starts with b2, b3");
 
 // Cooperative copy by the whole block; the caller synchronises before use.
 __device__ inline void stage_scalars(MlpScalars& dst, const MlpImage* img, const MlpNorm* norm) {
@@ -115,31 +121,85 @@ __device__ inline uint32_t c32_row(uint32_t i, uint32_t lane) {
   return 8u * (i >> 2) + 4u * (lane >> 5) + (i & 3u);
 }
 
-// Layer 1 of the calling lane's sample in FP32: relu(b1 + W1 x) rounded to bf16, into a[0..63]
-// (a row of a [s][h] tile) and, when hs is given, into column s of an [h][s] tile.
-__device__ inline void layer1(const MlpScalars& W, const float x[kMlpIn], __bf16* a, __bf16* hs,
-                              uint32_t s) {
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values rounded to bf16 (nearest even) in one packed conversion.
+__device__ inline uint32_t pack_bf16x2(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+// relu of two FP32 values as packed bf16: round first, then a signed 16-bit max against 0.
+__device__ inline uint32_t relu_bf16x2(float a, float b) {
+  const i16x2 q = __builtin_bit_cast(i16x2, pack_bf16x2(a, b));
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(q, (i16x2){0, 0}));
+}
+__device__ inline __bf16 bf16_half(uint32_t pair, uint32_t hi) {
+  return __builtin_bit_cast(__bf16, (unsigned short)(hi ? pair >> 16 : pair & 0xffffu));
+}
+
+// relu of a 32x32 accumulator tile (m = h0 + row, n = s0 + column) as bf16 into an [s][h] tile
+// (sh, 8-byte stores) and, when given, an [h][s] tile (hs).
+__device__ inline void store_relu(const f32x16& c, uint32_t h0, uint32_t s0, __bf16* sh,
+                                  __bf16* hs, uint32_t lane) {
+  const uint32_t s = s0 + (lane & 31);
 #pragma unroll
-  for (int g = 0; g < 8; ++g) {
-    bf16x8 r;
+  for (uint32_t g = 0; g < 4; ++g) {
+    const u32x2 r = {relu_bf16x2(c[4 * g], c[4 * g + 1]), relu_bf16x2(c[4 * g + 2], c[4 * g + 3])};
+    if (hs) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int h = g * 8 + j;
-      float acc = W.b1[h];
-#pragma unroll
-      for (int k = 0; k < kMlpIn; ++k) acc = __builtin_fmaf(W.w1[h * kMlpIn + k], x[k], acc);
-      r[j] = (__bf16)(acc > 0.0f ? acc : 0.0f);
-      if (hs) hs[h * kLdA + s] = r[j];
+      for (uint32_t q = 0; q < 4; ++q)
+        hs[(h0 + c32_row(4 * g + q, lane)) * kLdA + s] = bf16_half(r[q >> 1], q & 1u);
     }
-    *reinterpret_cast<bf16x8*>(a + g * 8) = r;
+    *reinterpret_cast<u32x2*>(sh + s * kLdA + h0 + c32_row(4 * g, lane)) = r;
   }
 }
 
-// Layer-2 pre-activations: acc[mt*2+nt] (h2 tile mt, sample tile nt) = W2 a1^T, a1 as [s][h1].
-__device__ inline void layer2(const MlpImage& W, const __bf16* a1_sh, f32x16 acc[4],
+// Layer 1 of a 64-sample tile: relu(W1 x + b1) as bf16 into act ([s][h1], row kLdA) and, when
+// given, an [h1][s] tile (hs).  The lane's input row (x hi words, lo words, 1, 1, 0...) passes
+// through elements 0..15 of act's row `lane`, which the outputs then overwrite.
+__device__ inline void layer1(const MlpImage& W, const float xn[kMlpIn], __bf16* act, __bf16* hs,
                               uint32_t lane) {
+  float hi[kMlpIn], lo[kMlpIn];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = f32x16{};
+  for (int k = 0; k < kMlpIn; ++k) {
+    hi[k] = (float)(__bf16)xn[k];
+    lo[k] = xn[k] - hi[k];  // exact; rounded to bf16 when packed
+  }
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  const uint32_t one = 0x3f803f80u;  // two bf16 1.0
+  const u32x4 r0 = {pack_bf16x2(hi[0], hi[1]), pack_bf16x2(hi[2], hi[3]),
+                    pack_bf16x2(hi[4], lo[0]), pack_bf16x2(lo[1], lo[2])};
+  const u32x4 r1 = {pack_bf16x2(lo[3], lo[4]), one, 0u, 0u};
+  *reinterpret_cast<u32x4*>(act + lane * kLdA) = r0;
+  *reinterpret_cast<u32x4*>(act + lane * kLdA + 8) = r1;
+  wave_lds_sync();
+  const bf16x8 x0 = ld8(act + (lane & 31) * kLdA + (lane >> 5) * 8);         // samples 0..31
+  const bf16x8 x1 = ld8(act + (32 + (lane & 31)) * kLdA + (lane >> 5) * 8);  // samples 32..63
+  wave_lds_sync();  // every lane holds its operands before the rows are overwritten
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt) {
+    const bf16x8 a = W.w1f[mt][lane];
+    const f32x16 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, x0, f32x16{}, 0, 0, 0);
+    const f32x16 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, x1, f32x16{}, 0, 0, 0);
+    store_relu(c0, mt * 32, 0, act, hs, lane);
+    store_relu(c1, mt * 32, 32, act, hs, lane);
+  }
+}
+
+// Layer-2 pre-activations: acc[mt*2+nt] (h2 tile mt, sample tile nt) = b2 + W2 a1^T, a1 as
+// [s][h1]; the bias initialises the accumulators.
+__device__ inline void layer2(const MlpImage& W, const float* b2, const __bf16* a1_sh,
+                              f32x16 acc[4], uint32_t lane) {
+#pragma unroll
+  for (uint32_t mt = 0; mt < 2; ++mt) {
+    f32x16 b;
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) b[i] = b2[mt * 32 + c32_row(i, lane)];
+    acc[mt * 2] = b;
+    acc[mt * 2 + 1] = b;
+  }
 #pragma unroll
   for (uint32_t ks = 0; ks < 4; ++ks)
 #pragma unroll
@@ -149,25 +209,6 @@ __device__ inline void layer2(const MlpImage& W, const __bf16* a1_sh, f32x16 acc
       for (uint32_t nt = 0; nt < 2; ++nt)
         mma32(acc[mt * 2 + nt], a, a1_sh + nt * 32 * kLdA + ks * 16, kLdA, lane);
     }
-}
-
-// relu(acc + bias[m]) of a 32x32 tile (m = h0 + row, n = s0 + column) as bf16 into an [s][h]
-// tile (sh, 8-byte stores) and, when given, an [h][s] tile (hs).
-__device__ inline void store_relu(const f32x16& c, const float* bias, uint32_t h0, uint32_t s0,
-                                  __bf16* sh, __bf16* hs, uint32_t lane) {
-  const uint32_t s = s0 + (lane & 31);
-#pragma unroll
-  for (uint32_t g = 0; g < 4; ++g) {
-    bf16x4 r;
-#pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) {
-      const uint32_t h = h0 + c32_row(4 * g + q, lane);
-      float x = c[4 * g + q] + bias[h];
-      r[q] = (__bf16)(x > 0.0f ? x : 0.0f);
-      if (hs) hs[h * kLdA + s] = r[q];
-    }
-    *reinterpret_cast<bf16x4*>(sh + s * kLdA + h0 + c32_row(4 * g, lane)) = r;
-  }
 }
 
 // LDS of one inference wavefront: [s][h] layer-1, then layer-2 activations; the 16 padding
@@ -183,16 +224,16 @@ __device__ inline f32x4* out_slot(InferLds& L, uint32_t s) {
 // wavefront must call it (the MFMAs use all 64 lanes).
 __device__ inline void mlp_infer(const MlpImage& W, const MlpScalars& S, InferLds& L,
                                  const float xn[kMlpIn], float y[kMlpOut], uint32_t lane) {
-  layer1(S, xn, L.act + lane * kLdA, nullptr, lane);
+  layer1(W, xn, L.act, nullptr, lane);
   wave_lds_sync();
   f32x16 acc[4];
-  layer2(W, L.act, acc, lane);
+  layer2(W, S.b2, L.act, acc, lane);
   wave_lds_sync();  // every lane has read the layer-1 activations
 #pragma unroll
   for (uint32_t mt = 0; mt < 2; ++mt)
 #pragma unroll
     for (uint32_t nt = 0; nt < 2; ++nt)
-      store_relu(acc[mt * 2 + nt], S.b2, mt * 32, nt * 32, L.act, nullptr, lane);
+      store_relu(acc[mt * 2 + nt], mt * 32, nt * 32, L.act, nullptr, lane);
   wave_lds_sync();
   const bf16x8 w30 = W.w3f[0][lane], w31 = W.w3f[1][lane];
   const bf16x8 w30l = W.w3lf[0][lane], w31l = W.w3lf[1][lane];
@@ -267,15 +308,15 @@ __device__ inline void mlp_train_chunk(const MlpImage& W, const MlpScalars& S, T
                                        GradAcc& G, const float xn[kMlpIn], float gscale,
                                        uint32_t lane) {
   // ---- forward ----
-  layer1(S, xn, L.u.f.a1_sh + lane * kLdA, L.a1_hs, lane);
+  layer1(W, xn, L.u.f.a1_sh, L.a1_hs, lane);
   wave_lds_sync();
   f32x16 acc[4];
-  layer2(W, L.u.f.a1_sh, acc, lane);
+  layer2(W, S.b2, L.u.f.a1_sh, acc, lane);
 #pragma unroll
   for (uint32_t mt = 0; mt < 2; ++mt)
 #pragma unroll
     for (uint32_t nt = 0; nt < 2; ++nt)
-      store_relu(acc[mt * 2 + nt], S.b2, mt * 32, nt * 32, L.u.f.a2_sh, L.a2_hs, lane);
+      store_relu(acc[mt * 2 + nt], mt * 32, nt * 32, L.u.f.a2_sh, L.a2_hs, lane);
   wave_lds_sync();
   {
     const bf16x8 w30 = W.w3f[0][lane], w31 = W.w3f[1][lane];

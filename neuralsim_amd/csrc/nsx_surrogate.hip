@@ -174,12 +174,18 @@ __global__ void k_surr_pack(const float* w, MlpImage* img) {
         const uint32_t mt = (f - 12) >> 2, ks = (f - 12) & 3u;
         const uint32_t h1 = mt * 32 + (l & 31), h2 = ks * 16 + (l >> 5) * 8 + j;
         x = (__bf16)w[kOffW2 + h2 * 64 + h1];
-      } else {  // W3^T [h2][o] hi (20, 21) and lo (22, 23) words: m = h2 (tile mt), k = o (< 4)
+      } else if (f < 24) {  // W3^T [h2][o] hi (20, 21) and lo (22, 23) words: m = h2, k = o (< 4)
         const uint32_t mt = (f - 20) & 1u, o = (l >> 5) * 8 + j;
         if (o < 4) {
           const float v = w[kOffW3 + o * 64 + mt * 32 + (l & 31)];
           x = f < 22 ? (__bf16)v : bf16_lo(v);
         }
+      } else {  // layer 1 [h1][k]: m = h1 (tile f - 24); k = input hi / lo column, bias words
+        const uint32_t h = (f - 24) * 32 + (l & 31), k = (l >> 5) * 8 + j;
+        if (k < (uint32_t)kL1Lo) x = (__bf16)w[kOffW1 + h * kMlpIn + k];
+        else if (k < (uint32_t)kL1Bias) x = (__bf16)w[kOffW1 + h * kMlpIn + k - kL1Lo];
+        else if (k == (uint32_t)kL1Bias) x = (__bf16)w[kOffB1 + h];
+        else if (k == (uint32_t)kL1Bias + 1) x = bf16_lo(w[kOffB1 + h]);
       }
       r[j] = x;
     }
@@ -188,14 +194,13 @@ __global__ void k_surr_pack(const float* w, MlpImage* img) {
     else if (f < 12) img->w3lf[f - 10][l] = r;
     else if (f < 20) img->w2tf[f - 12][l] = r;
     else if (f < 22) img->w3tf[f - 20][l] = r;
-    else img->w3tlf[f - 22][l] = r;
+    else if (f < 24) img->w3tlf[f - 22][l] = r;
+    else img->w1f[f - 24][l] = r;
     return;
   }
   const uint32_t u = t - (uint32_t)kImageFrags * 64u;
-  if (u < 320u) img->w1[u] = (float)(__bf16)w[kOffW1 + u];
-  else if (u < 384u) img->b1[u - 320] = w[kOffB1 + u - 320];
-  else if (u < 448u) img->b2[u - 384] = w[kOffB2 + u - 384];
-  else if (u < 452u) img->b3[u - 448] = w[kOffB3 + u - 448];
+  if (u < (uint32_t)kMlpH) img->b2[u] = w[kOffB2 + u];
+  else if (u < (uint32_t)kScalarHead) img->b3[u - kMlpH] = w[kOffB3 + u - kMlpH];
 }
 
 struct TrainArgs {
@@ -502,7 +507,7 @@ static nsx_status sur_parts(nsx_ctx* ctx, uint32_t waves) {
 }
 
 static hipError_t sur_pack(nsx_ctx* ctx) {
-  hipLaunchKernelGGL(k_surr_pack, dim3((kImageFrags * 64 + 452 + 255) / 256), dim3(256), 0,
+  hipLaunchKernelGGL(k_surr_pack, dim3((kImageFrags * 64 + kScalarHead + 255) / 256), dim3(256), 0,
                      ctx->stream,
                      ctx->sur.master, (MlpImage*)ctx->sur.image);
   return hipGetLastError();

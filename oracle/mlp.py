@@ -11,11 +11,14 @@ normalised quantities: x_n = (x - in_mu) * in_isd, target_n = (target - out_mu) 
 Loss = mean over samples and the 4 outputs of (y - target_n)^2.
 
 `emulate=True` reproduces the device's roundings: weights to bf16 (layers 1-2; the output
-layer's as the sum of two bf16 words, hi + bf16(W3 - hi)), the layer-1 and layer-2 activations
-to bf16, the input normalisation in FP32; everything else is FP64.
-`emulate="weights"` / `"act"` apply only the weight / only the activation roundings, "w1",
-"w2", "w3" only that layer's weight rounding.
-The backward pass treats the roundings as the identity (as the device does).
+layer's as the sum of two bf16 words, hi + bf16(W3 - hi)), the layer-1 bias as two bf16 words,
+the normalised inputs as two bf16 words (x = hi + bf16(x - hi), the layer-1 MFMA's operands),
+the layer-1 and layer-2 activations to bf16, the input normalisation in FP32; everything else
+is FP64.
+`emulate="weights"` / `"act"` apply only the weight (and bias) / only the input and activation
+roundings, "w1", "w2", "w3" only that layer's weight rounding.
+The backward pass treats the roundings as the identity (as the device does): the layer-1
+weight gradient uses the unsplit inputs.
 """
 import numpy as np
 
@@ -31,6 +34,13 @@ def round_bf16(x):
     b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
     b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
     return b.astype(np.uint32).view(np.float32)
+
+
+def split_bf16(x):
+    """x as the sum of two bf16 words, hi + bf16(x - hi) (the difference is exact in FP32)."""
+    x = np.asarray(x, np.float32)
+    hi = round_bf16(x)
+    return hi.astype(np.float64) + round_bf16((x - hi).astype(np.float32))
 
 
 def unpack(params):
@@ -79,6 +89,8 @@ def _weights(params, emulate):
                 hi = round_bf16(v)
                 # the output layer carries a second bf16 word: W3 = hi + bf16(W3 - hi)
                 v = (hi.astype(np.float64) + round_bf16(v - hi)) if k == "w3" else hi
+            if k == "b1" and emulate in (True, "weights", "w1"):
+                v = split_bf16(v)  # two bf16 words against constant-1 inputs
         out[k] = np.asarray(v, np.float64)
     return out
 
@@ -86,8 +98,9 @@ def _weights(params, emulate):
 def forward(params, xn, emulate=True):
     """Normalised outputs (count x 4) and the activations, from normalised inputs xn."""
     w = _weights(params, emulate)
+    xe = split_bf16(xn) if emulate in (True, "act") else np.asarray(xn, np.float64)
     xn = np.asarray(xn, np.float64)
-    z1 = xn @ w["w1"].T + w["b1"]
+    z1 = xe @ w["w1"].T + w["b1"]
     a1 = np.maximum(z1, 0.0)
     if emulate in (True, "act"):
         a1 = round_bf16(a1.astype(np.float32)).astype(np.float64)

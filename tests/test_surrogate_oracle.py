@@ -32,6 +32,21 @@ def test_round_bf16_matches_torch():
     assert np.array_equal(mlp.round_bf16(x), ref)
 
 
+def test_split_bf16_two_words():
+    """Layer-1 operands: x = hi + bf16(x - hi).  A hand-worked case, then the error bound of two
+    bf16 words (each rounding is within 2^-8 relative, so 2^-16 |x|), which a split that
+    dropped the low word (error up to 2^-8) fails."""
+    x = np.float32(1.0 + 2.0**-8 + 2.0**-12)  # above the tie: hi = 1 + 2^-7
+    assert mlp.round_bf16(x) == np.float32(1.0 + 2.0**-7)
+    assert mlp.split_bf16(x) == 1.0 + 2.0**-8 + 2.0**-12  # lo = -(2^-8 - 2^-12), exact
+    rng = np.random.default_rng(5)
+    v = (rng.standard_normal(200_000) * np.exp(rng.uniform(-8, 8, 200_000))).astype(np.float32)
+    rel = np.abs(mlp.split_bf16(v) - v.astype(np.float64)) / np.abs(v.astype(np.float64))
+    assert rel.max() <= 2.0**-16 and rel.max() > 2.0**-19
+    one_word = np.abs(mlp.round_bf16(v).astype(np.float64) - v) / np.abs(v.astype(np.float64))
+    assert one_word.max() > 2.0**-10
+
+
 def test_backward_matches_finite_differences():
     rng = np.random.default_rng(1)
     p = mlp.init_params(3).astype(np.float64)

@@ -23,15 +23,18 @@ def log(*a):
 
 
 def neuron_bytes(cfg, n, spikes):
-    """Algorithmic bytes of one update launch: per-neuron state read+write, ring slot read+clear,
-    plus per-spike state (jitter c/d, count, last step, pre trace record, spike list)."""
+    """Algorithmic bytes of one update launch, a lower bound: per-neuron state read and the
+    writes that always happen, ring slots read, plus per-spike state (jitter c/d, count, last
+    step, pre trace record, spike list).  Writes the kernel skips when the value's bits do not
+    change (a ring slot that received nothing, LIF currents that stay zero, a V at rest) are not
+    counted."""
     from neuralsim_amd import abi
     if cfg.model == abi.MODEL_IZHIKEVICH:
-        per_n, per_s = 8 + 8 + 16, 8 + 8 + 4 + 16 + 4
+        per_n, per_s = 8 + 8 + 8, 8 + 8 + 4 + 16 + 4  # v, u read+write; ring read
     elif cfg.model == abi.MODEL_LIF:
-        per_n, per_s = 8 + 8 + 8 + 8 + 32, 8 + 4 + 4
+        per_n, per_s = 16 + 16 + 4, 8 + 4 + 4  # v, 2 currents, counter, 2 ring slots read; v write
     else:
-        per_n, per_s = 4 * 8 * 2 + 32, 8 + 4 + 4
+        per_n, per_s = 4 * 8 * 2 + 16, 8 + 4 + 4  # 8 state words read+write; 2 ring slots read
     return per_n * n + per_s * spikes
 
 

@@ -22,19 +22,15 @@ struct U4 {
 };
 
 // Philox4x32-10 (Salmon et al., SC'11): 10 rounds of two 32x32->64-bit products with a
-// Weyl-sequence key schedule.
+// Weyl-sequence key schedule.  Each product is one 64-bit multiply (v_mad_u64_u32 on the device,
+// one instruction for both halves).
 __host__ __device__ inline U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                      uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-#else
-    uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
-    uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
-    uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-#endif
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
     c0 = n0;
     c1 = lo1;

@@ -90,18 +90,39 @@ __device__ inline void hh_frozen(float& v, float& m, float& h, float& n, float I
   for (uint32_t k = 0; k < substeps; ++k) hh_apply(v, m, h, n, I, r, c);
 }
 
-// Synaptic conductances and external drive of neuron i for one step; returns the input current
-// held over the step's sub-steps.  AMPA / NMDA open on excitatory arrivals a0 (NMDA through the
-// Mg block at the step-start V, synapse/mod.rs:41-47), GABA-A on inhibitory arrivals |a1|; the
-// currents use the step-start V.  OU drive: the current value is used and the process advanced
-// by its exact one-step transition with the normal draw z.
+// Conductances (AMPA, GABA-A, NMDA) and OU drive value of an HH neuron, as loaded from and
+// stored to s0, s1, s2, drv.
+struct HHSyn {
+  float ga, gg, gn, drv;
+};
+
 template <int DRIVE>
-__device__ inline float hh_input(const StepParams& p, const DevPtrs& s, uint32_t i, float v,
-                                 float a0, float a1, float z) {
-  float ga = s.s0[i] * p.dec_e;
+__device__ inline HHSyn hh_syn_load(const DevPtrs& s, uint32_t i) {
+  HHSyn y{s.s0[i], s.s1[i], s.s2[i], 0.0f};
+  if (DRIVE == NSX_DRIVE_OU) y.drv = s.drv[i];
+  return y;
+}
+
+template <int DRIVE>
+__device__ inline void hh_syn_store(const DevPtrs& s, uint32_t i, const HHSyn& y) {
+  s.s0[i] = y.ga;
+  s.s1[i] = y.gg;
+  s.s2[i] = y.gn;
+  if (DRIVE == NSX_DRIVE_OU) s.drv[i] = y.drv;
+}
+
+// Synaptic conductances and external drive of a neuron for one step (y updated in place);
+// returns the input current held over the step's sub-steps.  AMPA / NMDA open on excitatory
+// arrivals a0 (NMDA through the Mg block at the step-start V, synapse/mod.rs:41-47), GABA-A on
+// inhibitory arrivals |a1|; the currents use the step-start V.  OU drive: the current value is
+// used and the process advanced by its exact one-step transition with the normal draw z.
+template <int DRIVE>
+__device__ inline float hh_input(const StepParams& p, HHSyn& y, float v, float a0, float a1,
+                                 float z) {
+  float ga = y.ga * p.dec_e;
   float xa = a0 * p.gmax_ampa;
   ga = ga + xa;
-  float gn = s.s2[i] * p.dec_n;
+  float gn = y.gn * p.dec_n;
   if (p.use_nmda) {
     float e = expf_(-0.062f * v);
     float bl = p.mgc * e;
@@ -110,7 +131,7 @@ __device__ inline float hh_input(const StepParams& p, const DevPtrs& s, uint32_t
     xn = xn / bl;
     gn = gn + xn;
   }
-  float gg = s.s1[i] * p.dec_i;
+  float gg = y.gg * p.dec_i;
   float xg = fabsf(a1) * p.gmax_gaba;
   gg = gg + xg;
   float de = p.e_exc - v, di = p.e_inh - v;
@@ -121,18 +142,18 @@ __device__ inline float hh_input(const StepParams& p, const DevPtrs& s, uint32_t
   isyn = isyn + tg;
   float iext = 0.0f;
   if (DRIVE == NSX_DRIVE_OU) {
-    iext = s.drv[i];
+    iext = y.drv;
     float o = iext * p.ou_k1;
     o = o + p.ou_k2;
     float q = p.ou_k3 * z;
-    s.drv[i] = o + q;
+    y.drv = o + q;
   } else if (DRIVE == NSX_DRIVE_GAUSSIAN) {
     float q = p.drive_sigma * z;
     iext = p.drive_mean + q;
   }
-  s.s0[i] = ga;
-  s.s1[i] = gg;
-  s.s2[i] = gn;
+  y.ga = ga;
+  y.gg = gg;
+  y.gn = gn;
   return iext + isyn;
 }
 

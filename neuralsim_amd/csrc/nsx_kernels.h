@@ -63,49 +63,84 @@ __device__ inline NeuronId neuron_id(const StepParams& p, const DevPtrs& s, uint
   return NeuronId{b.seed, i - b.offset, b.n, b.n_exc};
 }
 
-// Per-step drive value of a neuron (value added to channel 0 / the input current).
+// Drive words of one step for a block's neurons.  One Philox draw covers four consecutive
+// neurons of a network (local indices 4g .. 4g+3; blocks start at a multiple of 256 of their
+// network): wavefront 0 draws the block's 64 groups into w[0 .. kBlock).  The caller synchronises
+// the block before drive_draw reads them.
 template <int DRIVE>
-__device__ inline void drive_draw(const StepParams& p, const NeuronId& id, int32_t t, float& a0,
-                                  float& z) {
+__device__ inline void drive_stage(const StepParams& p, const DevPtrs& s, int32_t t, uint32_t* w) {
   if (DRIVE == NSX_DRIVE_NONE) return;
-  U4 r = draw(id.seed, id.local, (uint32_t)t, kStreamDrive);
+  if (threadIdx.x < kBlock / 4) {
+    const NeuronId id0 = neuron_id(p, s, blockIdx.x * kBlock, blockIdx.x);
+    const U4 r = draw(id0.seed, (uint64_t)(id0.local >> 2) + threadIdx.x, (uint32_t)t,
+                      kStreamDrive);
+    w[4 * threadIdx.x] = r.x;
+    w[4 * threadIdx.x + 1] = r.y;
+    w[4 * threadIdx.x + 2] = r.z;
+    w[4 * threadIdx.x + 3] = r.w;
+  }
+}
+
+// Per-step drive value of the calling thread's neuron from the staged words (value added to
+// channel 0 / the input current).  Poisson: word (local & 3) by inversion.  Gaussian: the words
+// 2k, 2k+1 of pair k = (local & 3) / 2 by Box-Muller (1958), the cosine for the even neuron and
+// the sine for the odd one -- the angle word advanced by three quarter turns, exact on the 24-bit
+// grid of the angle.
+template <int DRIVE>
+__device__ inline void drive_draw(const StepParams& p, const uint32_t* w, float& a0, float& z) {
+  if (DRIVE == NSX_DRIVE_NONE) return;
+  const uint32_t k = threadIdx.x;
   if (DRIVE == NSX_DRIVE_POISSON) {
+    const uint32_t x = w[k];
     uint32_t c = 0;
-    for (int k = 0; k < p.npthr; ++k) c += (r.x >= p.pthr[k]) ? 1u : 0u;
-    float x = (float)c * p.drive_weight;
-    a0 = a0 + x;
+    for (int q = 0; q < p.npthr; ++q) c += (x >= p.pthr[q]) ? 1u : 0u;
+    float f = (float)c * p.drive_weight;
+    a0 = a0 + f;
   } else {
-    z = normal_(r.x, r.y);
+    const uint32_t b = (k & ~3u) | (k & 2u);
+    const uint32_t angle = w[b + 1] + ((k & 1u) ? 0xC0000000u : 0u);
+    z = normal_(w[b], angle);
   }
 }
 
-// First thing of every update kernel with pair STDP: the post-synaptic trace bumps of this
-// block's spikes at step t-1 (its segment still holds them; it is rewritten at the end).
+// Pair STDP: the post-synaptic trace bump of neuron i's spike at step t-1, deferred to the update
+// of step t so that the synapse pass of step t-1 read the record without it.  `last` is the
+// neuron's last spike step as loaded at the start of the update (not loaded, and kNeverStep, in a
+// block whose spike segment of step t-1 is empty); each thread bumps its own neuron, before any
+// spike_record of step t.
 template <bool PAIR>
-__device__ inline void update_prologue(const StepParams& p, const DevPtrs& s, int32_t t) {
-  if (!PAIR) return;
-  const uint32_t prev = s.bcnt[blockIdx.x];
-  if (threadIdx.x < prev) {
-    const uint32_t j = s.spikes[blockIdx.x * kBlock + threadIdx.x];
-    if (j < p.n_exc) s.post[j] = post_bump(s.post[j], s.mtab, p.mtab_n, t - 1);
-  }
-  __syncthreads();
+__device__ inline void post_bump_prev(const StepParams& p, const DevPtrs& s, uint32_t i,
+                                      int32_t last, int32_t t) {
+  if (PAIR && last == t - 1 && i < p.n_exc)
+    s.post[i] = post_bump(s.post[i], s.mtab, p.mtab_n, t - 1);
 }
 
-// Read and clear neuron i's ring slot of step t: channel 0 into a0, channel 1 into a1 when the
-// network has two channels (TWO).
+// Neuron i's ring slot of step t: ring_load reads channel 0 (and channel 1 when the network has
+// two, TWO), ring_take later converts and clears -- a slot that received nothing is not
+// rewritten.  Split so that the update kernels issue all of a neuron's loads at once.
+struct RingIn {
+  long long* r0;
+  long long q0, q1;
+};
+
 template <bool TWO>
-__device__ inline void ring_take(const StepParams& p, const DevPtrs& s, uint32_t i, int32_t t,
-                                 float& a0, float& a1) {
+__device__ inline RingIn ring_load(const StepParams& p, const DevPtrs& s, uint32_t i, int32_t t) {
   const uint32_t slot = (uint32_t)t % p.D;
-  long long* r0 = s.ring + (size_t)slot * p.ch * p.n + i;
-  a0 = dequantize(*r0, p.scale_out);
-  *r0 = 0;
+  RingIn r;
+  r.r0 = s.ring + (size_t)slot * p.ch * p.n + i;
+  r.q0 = r.r0[0];
+  r.q1 = TWO ? r.r0[p.n] : 0;
+  return r;
+}
+
+template <bool TWO>
+__device__ inline void ring_take(const StepParams& p, const RingIn& r, float& a0, float& a1) {
+  a0 = dequantize(r.q0, p.scale_out);
+  if (r.q0 != 0) r.r0[0] = 0;
   a1 = 0.0f;
   if (TWO) {
-    long long* r1 = r0 + p.n;
-    a1 = dequantize(*r1, p.scale_out);
-    *r1 = 0;
+    a1 = dequantize(r.q1, p.scale_out);
+    if (r.q1 != 0) r.r0[p.n] = 0;
   }
 }
 

@@ -31,33 +31,63 @@ namespace nsx {
 
 template <int MODEL, int DRIVE, int PL>
 __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
+  constexpr bool TWO = MODEL != NSX_MODEL_IZHIKEVICH;
+  __shared__ uint32_t s_drive[kBlock];
   const int32_t t = *s.tbase + off;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  update_prologue<PL == NSX_PLAST_PAIR>(p, s, t);
   const NeuronId id = neuron_id(p, s, i, blockIdx.x);
-  bool spiked = false;
-  if (i < p.n && id.local < id.n) {  // batch padding neurons stay idle
-    float a0, a1;
-    ring_take<MODEL != NSX_MODEL_IZHIKEVICH>(p, s, i, t, a0, a1);
-    float z = 0.0f;
-    drive_draw<DRIVE>(p, id, t, a0, z);
+  const bool live = i < p.n && id.local < id.n;  // batch padding neurons stay idle
+  // Every load of the neuron's update is issued here, ahead of the block's drive staging and
+  // barrier, so that they are in flight together (one memory round trip, not one per field).
+  constexpr bool PAIR = PL == NSX_PLAST_PAIR;
+  RingIn rg{nullptr, 0, 0};
+  float v = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;  // LIF: v ie ii; Izh: v u; HH: v m h n
+  int32_t refr = 0, last = kNeverStep;
+  HHSyn y{0.0f, 0.0f, 0.0f, 0.0f};
+  // blocks with no spike at t-1 (their segment count is 0) skip the last-spike loads
+  const uint32_t prev = PAIR ? s.bcnt[blockIdx.x] : 0u;
+  if (live) {
+    rg = ring_load<TWO>(p, s, i, t);
+    v = s.v[i];
+    if (PAIR && prev) last = s.last[i];
     if (MODEL == NSX_MODEL_LIF) {
-      float ie = s.s0[i] * p.dec_e;
+      w1 = s.s0[i];
+      w2 = s.s1[i];
+      refr = s.refr[i];
+    } else if (MODEL == NSX_MODEL_IZHIKEVICH) {
+      w1 = s.u[i];
+    } else {
+      w1 = s.gm[i];
+      w2 = s.gh[i];
+      w3 = s.gn[i];
+      y = hh_syn_load<DRIVE>(s, i);
+    }
+  }
+  drive_stage<DRIVE>(p, s, t, s_drive);
+  if (DRIVE != NSX_DRIVE_NONE) __syncthreads();  // the staged words
+  bool spiked = false;
+  if (live) {
+    post_bump_prev<PAIR>(p, s, i, last, t);
+    float a0, a1;
+    ring_take<TWO>(p, rg, a0, a1);
+    float z = 0.0f;
+    drive_draw<DRIVE>(p, s_drive, a0, z);
+    if (MODEL == NSX_MODEL_LIF) {
+      // values whose bits do not change are not rewritten (silent neurons: decayed currents)
+      float ie = w1 * p.dec_e;
       ie = ie + a0;
-      float ii = s.s1[i] * p.dec_i;
+      float ii = w2 * p.dec_i;
       ii = ii + a1;
-      s.s0[i] = ie;
-      s.s1[i] = ii;
-      int32_t refr = s.refr[i];
+      if (__float_as_uint(ie) != __float_as_uint(w1)) s.s0[i] = ie;
+      if (__float_as_uint(ii) != __float_as_uint(w2)) s.s1[i] = ii;
       if (refr > 0) {
         s.refr[i] = refr - 1;
       } else {
-        float v = s.v[i];
         float I = ie + ii;
         float x = v - p.v_rest;
         x = -x;
-        float y = p.lif_r * I;
-        x = x + y;
+        float yv = p.lif_r * I;
+        x = x + yv;
         x = x * p.lif_k;
         float vn = v + x;
         if (vn >= p.v_th) {
@@ -65,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
           s.refr[i] = p.refr_steps;
           spiked = true;
         }
-        s.v[i] = vn;
+        if (__float_as_uint(vn) != __float_as_uint(v)) s.v[i] = vn;
       }
     } else if (MODEL == NSX_MODEL_IZHIKEVICH) {
       float I = a0;
@@ -74,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
         q = p.drive_mean + q;
         I = I + q;
       }
-      float v = s.v[i], u = s.u[i];
+      float u = w1;
       if (v >= 30.0f) {
         v = s.c[i];
         u = u + s.d[i];
@@ -103,9 +133,9 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       s.v[i] = v;
       s.u[i] = u;
     } else {  // Hodgkin-Huxley with AMPA / GABA-A / NMDA conductances
-      const float v = s.v[i];
-      const float I = hh_input<DRIVE>(p, s, i, v, a0, a1, z);  // held over the sub-steps
-      float m = s.gm[i], h = s.gh[i], nn = s.gn[i];
+      const float I = hh_input<DRIVE>(p, y, v, a0, a1, z);  // held over the sub-steps
+      hh_syn_store<DRIVE>(s, i, y);
+      float m = w1, h = w2, nn = w3;
       float vn = v;
       for (uint32_t k = 0; k < p.hh_substeps; ++k) hh_substep(vn, m, h, nn, I, p.hh);
       if (p.hh_reset_mode) {

@@ -399,28 +399,42 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
   __shared__ InferLds L[kBlock / 64];
   __shared__ MlpScalars S;
   __shared__ uint32_t s_fb[kBlock / 64];
+  __shared__ uint32_t s_drive[kBlock];
   const int32_t t = *s.tbase + off;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  stage_scalars(S, q.img, q.norm);
-  update_prologue<PAIR>(p, s, t);
-  __syncthreads();
-  const MlpNorm& nm = S.norm;
   const NeuronId id = neuron_id(p, s, i, blockIdx.x);
   const bool live = i < p.n && id.local < id.n;
+  // the neuron's loads first, in flight together while the block stages its scalars and drive
+  RingIn rg{nullptr, 0, 0};
+  HHSyn y{0.0f, 0.0f, 0.0f, 0.0f};
   float x[kMlpIn];
-#pragma unroll
-  for (int k = 0; k < kMlpIn; ++k) x[k] = nm.in_mu[k];
+  int32_t last = kNeverStep;
+  const uint32_t prev = PAIR ? s.bcnt[blockIdx.x] : 0u;  // 0: no spike at t-1 in the block
   if (live) {
-    float a0, a1;
-    ring_take<true>(p, s, i, t, a0, a1);
-    float z = 0.0f;
-    drive_draw<DRIVE>(p, id, t, a0, z);
+    rg = ring_load<true>(p, s, i, t);
+    if (PAIR && prev) last = s.last[i];
     x[0] = s.v[i];
-    x[4] = hh_input<DRIVE>(p, s, i, x[0], a0, a1, z);
     x[1] = s.gm[i];
     x[2] = s.gh[i];
     x[3] = s.gn[i];
+    y = hh_syn_load<DRIVE>(s, i);
+  }
+  stage_scalars(S, q.img, q.norm);
+  drive_stage<DRIVE>(p, s, t, s_drive);
+  __syncthreads();
+  const MlpNorm& nm = S.norm;
+  if (live) {
+    post_bump_prev<PAIR>(p, s, i, last, t);
+    float a0, a1;
+    ring_take<true>(p, rg, a0, a1);
+    float z = 0.0f;
+    drive_draw<DRIVE>(p, s_drive, a0, z);
+    x[4] = hh_input<DRIVE>(p, y, x[0], a0, a1, z);
+    hh_syn_store<DRIVE>(s, i, y);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) x[k] = nm.in_mu[k];
   }
   float nx[4], res[kMlpOut];
   const bool in = surr_step(*q.img, S, L[wid], p.hh, p.hh_substeps, x, live, nx, res, lane);

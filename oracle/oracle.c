@@ -170,6 +170,32 @@ static inline void draw4(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, u
   orc_philox4x32_10(ctr, key, out);
 }
 
+/* Drive words of neuron i at step t: one Philox draw per four consecutive neurons (group i/4).
+   Poisson takes word i%4; Gaussian the pair (i%4)/2, the angle word advanced by three quarter
+   turns for odd i (Box-Muller's sine of the pair; the device's drive_stage / drive_draw). */
+static inline uint32_t drive_word(uint64_t seed, uint64_t i, uint32_t t) {
+  uint32_t rr[4];
+  const uint64_t g = i >> 2;
+  draw4(seed, (uint32_t)g, t, STREAM_DRIVE, (uint32_t)(g >> 32), rr);
+  return rr[i & 3];
+}
+static inline void drive_pair(uint64_t seed, uint64_t i, uint32_t t, uint32_t* radius,
+                              uint32_t* angle) {
+  uint32_t rr[4];
+  const uint64_t g = i >> 2;
+  draw4(seed, (uint32_t)g, t, STREAM_DRIVE, (uint32_t)(g >> 32), rr);
+  const uint32_t b = (uint32_t)(i & 2);
+  *radius = rr[b];
+  *angle = rr[b + 1] + ((i & 1) ? 0xC0000000u : 0u);
+}
+
+/* The drive normal of neuron i at step t (FP32, as the mirror and the device draw it). */
+float orc_drive_normal(uint64_t seed, uint64_t i, uint32_t t) {
+  uint32_t ra, an;
+  drive_pair(seed, i, t, &ra, &an);
+  return orc_normal(ra, an);
+}
+
 /* ===================================================================================== */
 /* FP64 restatement of the reference                                                      */
 /* ===================================================================================== */
@@ -518,14 +544,14 @@ uint64_t ref_net_run(ref_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* 
       double a1 = ch > 1 ? r->ring[((size_t)slot * ch + 1) * n + i] : 0.0;
       r->ring[((size_t)slot * ch + 0) * n + i] = 0.0;
       if (ch > 1) r->ring[((size_t)slot * ch + 1) * n + i] = 0.0;
-      uint32_t rr[4];
       double z = 0.0, pois = 0.0;
-      if (cfg->drive != NSX_DRIVE_NONE) {
-        draw4(cfg->seed, (uint32_t)i, (uint32_t)t, STREAM_DRIVE, (uint32_t)(i >> 32), rr);
-        if (cfg->drive == NSX_DRIVE_POISSON)
-          pois = (double)poisson_draw(rr[0], r->pthr, r->npthr) * cfg->drive_weight;
-        else
-          z = orc_normal64(rr[0], rr[1]);
+      if (cfg->drive == NSX_DRIVE_POISSON) {
+        pois = (double)poisson_draw(drive_word(cfg->seed, i, (uint32_t)t), r->pthr, r->npthr) *
+               cfg->drive_weight;
+      } else if (cfg->drive != NSX_DRIVE_NONE) {
+        uint32_t ra, an;
+        drive_pair(cfg->seed, i, (uint32_t)t, &ra, &an);
+        z = orc_normal64(ra, an);
       }
       int spiked = 0;
       if (cfg->model == NSX_MODEL_LIF) {
@@ -1003,17 +1029,15 @@ uint64_t mir_run(mir_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* sp_i
         a1 = (float)((double)(*p1) * r->scale_out);
         *p1 = 0;
       }
-      uint32_t rr[4] = {0, 0, 0, 0};
       float z = 0.0f;
-      if (cfg->drive != NSX_DRIVE_NONE) {
-        draw4(cfg->seed, (uint32_t)i, (uint32_t)t, STREAM_DRIVE, (uint32_t)(i >> 32), rr);
-        if (cfg->drive == NSX_DRIVE_POISSON) {
-          float k = (float)poisson_draw(rr[0], r->pthr, r->npthr);
-          float x = k * cfg->drive_weight;
-          a0 = a0 + x;
-        } else {
-          z = orc_normal(rr[0], rr[1]);
-        }
+      if (cfg->drive == NSX_DRIVE_POISSON) {
+        float k = (float)poisson_draw(drive_word(cfg->seed, i, (uint32_t)t), r->pthr, r->npthr);
+        float x = k * cfg->drive_weight;
+        a0 = a0 + x;
+      } else if (cfg->drive != NSX_DRIVE_NONE) {
+        uint32_t ra, an;
+        drive_pair(cfg->seed, i, (uint32_t)t, &ra, &an);
+        z = orc_normal(ra, an);
       }
       int spiked = 0;
       if (cfg->model == NSX_MODEL_LIF) {

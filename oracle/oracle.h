@@ -26,6 +26,7 @@ float orc_logf(float x);
 float orc_cospi(float x);
 float orc_normal(uint32_t r0, uint32_t r1);       /* Box-Muller, float, own math        */
 double orc_normal64(uint32_t r0, uint32_t r1);    /* same uniforms, libm double math     */
+float orc_drive_normal(uint64_t seed, uint64_t i, uint32_t t); /* Gaussian drive of neuron i */
 int orc_poisson_table(double lambda, uint32_t* thr, int cap); /* returns entries        */
 /* FP32 mirror of one step's HH membrane integration of a single neuron (crossing mode): plain
  * Euler sub-steps, or the surrogate's sub-steps with rates frozen at the step-start V. */

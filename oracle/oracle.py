@@ -25,6 +25,7 @@ PROTOS = {
     "orc_cospi": (_f32, [_f32]),
     "orc_normal": (_f32, [_u32, _u32]),
     "orc_normal64": (_f64, [_u32, _u32]),
+    "orc_drive_normal": (_f32, [_u64, _u64, _u32]),
     "orc_poisson_table": (_int, [_f64, _P, _int]),
     "orc_hh_plain": (None, [C.POINTER(abi.NsxConfig), _P, _P, _P, _P, _f32, _u32]),
     "orc_hh_frozen": (None, [C.POINTER(abi.NsxConfig), _P, _P, _P, _P, _f32, _u32]),

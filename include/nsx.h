@@ -227,6 +227,35 @@ nsx_status nsx_read_raster(nsx_ctx* ctx, uint32_t* steps, uint32_t* neurons, uin
 /* Per-step spike counts of the last nsx_step call (cap entries at most). */
 nsx_status nsx_read_step_counts(nsx_ctx* ctx, uint32_t* counts, uint64_t cap, uint64_t* n_out);
 
+/* ---- Simulation statistics (SimulationStats, src/simulation/engine.rs:21-33, filled by
+ * SimulationEngine::step phase 3, engine.rs:617-670) ---------------------------------------- */
+typedef struct nsx_sim_stats {
+  uint64_t total_spikes;     /* since creation / the last reset                              */
+  uint64_t output_spikes;    /* 0: the device engine has no output population                */
+  uint64_t active_neurons;   /* largest spike count of one step (since stats were enabled)   */
+  uint64_t weight_updates;   /* spikes of the last step (the reference assigns exactly that) */
+  double sim_time_ms;
+  double mean_firing_rate;   /* Hz: mean over neurons of the low-pass rate r += (1000/dt *
+                              * spiked - r) * dt/100, updated every step                     */
+  double synchrony_index;    /* fraction of the neurons that fired in the last step          */
+  double weight_mean;        /* over all synapses, two-pass FP64, fixed summation order      */
+  double weight_std;         /* population standard deviation                                */
+  double lfp;                /* SimulationEngine::lfp (engine.rs:492, 804-806): running sum  *
+                              * of |synaptic currents| of the conductance (HH) models; 0     *
+                              * for the current-based LIF and Izhikevich synapses            */
+  uint64_t seed;
+  uint64_t steps;            /* steps folded since stats were enabled                        */
+} nsx_sim_stats;
+
+/* Enable (1) or disable (0) the statistics; enabling clears them.  While on, every step adds
+ * its spike count to a per-step counter and the HH kernels add |synaptic currents| per block;
+ * nsx_step folds the counts after each launch chunk. */
+nsx_status nsx_set_stats(nsx_ctx* ctx, int enable);
+/* The current SimulationStats (stats on); computes the weight statistics on the device. */
+nsx_status nsx_read_sim_stats(nsx_ctx* ctx, nsx_sim_stats* out);
+/* Mean and population standard deviation of all synaptic weights (engine.rs:655-666). */
+nsx_status nsx_weight_stats(nsx_ctx* ctx, double* mean, double* stddev);
+
 /* Kernel timing: when enabled, every launch of the named kernel class in nsx_step is bracketed
  * by hipEvents on the stream it runs on (steps run as direct launches, not graphs).  Enabling
  * starts a fresh measurement; disabling keeps it readable.  kind: 0 neuron update, 1 synapse

@@ -55,6 +55,9 @@ PROTOTYPES = {
     "nsx_set_prop_split": (_int, [_VP, _u32]),
     "nsx_read_raster": (_int, [_VP, _P, _P, _u64, C.POINTER(_u64), _int]),
     "nsx_read_step_counts": (_int, [_VP, _P, _u64, C.POINTER(_u64)]),
+    "nsx_set_stats": (_int, [_VP, _int]),
+    "nsx_read_sim_stats": (_int, [_VP, C.POINTER(abi.NsxSimStats)]),
+    "nsx_weight_stats": (_int, [_VP, C.POINTER(_f64), C.POINTER(_f64)]),
     "nsx_set_profiling": (_int, [_VP, _int]),
     "nsx_kernel_time": (_int, [_VP, _int, C.POINTER(_f64), C.POINTER(_u64), C.POINTER(_u64)]),
     "nsx_backend_new": (_int, [_u64, _u64, C.POINTER(_VP)]),

@@ -108,6 +108,24 @@ class NsxStepStats(C.Structure):
     ]
 
 
+class NsxSimStats(C.Structure):
+    """nsx_sim_stats (include/nsx.h): SimulationStats, src/simulation/engine.rs:21-33."""
+    _fields_ = [
+        ("total_spikes", C.c_uint64),
+        ("output_spikes", C.c_uint64),
+        ("active_neurons", C.c_uint64),
+        ("weight_updates", C.c_uint64),
+        ("sim_time_ms", C.c_double),
+        ("mean_firing_rate", C.c_double),
+        ("synchrony_index", C.c_double),
+        ("weight_mean", C.c_double),
+        ("weight_std", C.c_double),
+        ("lfp", C.c_double),
+        ("seed", C.c_uint64),
+        ("steps", C.c_uint64),
+    ]
+
+
 class NsxTuneReport(C.Structure):
     """nsx_tune_report (include/nsx.h)."""
     _fields_ = [

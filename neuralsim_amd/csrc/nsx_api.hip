@@ -262,7 +262,7 @@ static nsx_status check_device(nsx_ctx* ctx, int* dev) {
   return NSX_OK;
 }
 
-static void destroy_graph(nsx_ctx* ctx) {
+void destroy_graph(nsx_ctx* ctx) {
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   ctx->graph_exec = nullptr;
@@ -461,7 +461,10 @@ nsx_status nsx_reset_state(nsx_ctx* ctx) {
   NSX_HIP(ctx, hipMemsetAsync(ctx->d.tbase, 0, 64, ctx->stream));
   if (ctx->sur.fb)
     NSX_HIP(ctx, hipMemsetAsync(ctx->sur.fb, 0, (size_t)ctx->p.nblk * 8, ctx->stream));
+  if (ctx->d.lfp_part)
+    NSX_HIP(ctx, hipMemsetAsync(ctx->d.lfp_part, 0, (size_t)ctx->p.nblk * 8, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->hs = {};
   ctx->t = 0;
   ctx->tbase_host = 0;
   return NSX_OK;
@@ -719,6 +722,32 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   if (ctx->t + n_steps >= (1ull << 30)) return set_error(ctx, NSX_E_INVALID, "step index limit");
   if (ctx->cfg.model == NSX_MODEL_HH_SURROGATE && !(ctx->sur.has_params && ctx->sur.has_norm))
     return set_error(ctx, NSX_E_STATE, "surrogate parameters not set (train or load them)");
+  if (ctx->p.stats_on && n_steps > ctx->p.stat_cap) {
+    // the per-step count ring holds stat_cap steps: run in chunks, folding after each
+    nsx_step_stats one{};
+    uint64_t left = n_steps, spikes = 0, events = 0;
+    double wall = 0.0;
+    while (left) {
+      const uint64_t k = std::min<uint64_t>(left, ctx->p.stat_cap);
+      nsx_status s = nsx_step(ctx, k, &one);
+      if (s) return s;
+      spikes += one.spikes;
+      events += one.events;
+      wall += one.wall_ms;
+      left -= k;
+    }
+    if (st) {
+      *st = one;
+      st->steps_done = n_steps;
+      st->spikes = spikes;
+      st->events = events;
+      st->wall_ms = wall;
+    }
+    return NSX_OK;
+  }
+  if (ctx->p.stats_on && n_steps)
+    NSX_HIP(ctx, hipMemsetAsync(ctx->d.stat_counts, 0,
+                                (size_t)ctx->p.stat_cap * kStatStripes * 4, ctx->stream));
   uint64_t c0[kNumPart];
   nsx_status s0 = read_totals(ctx, c0);
   if (s0) return s0;
@@ -794,6 +823,10 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   if (rast > ctx->p.raster_cap && ctx->cfg.record_raster)
     set_error(ctx, NSX_E_OVERFLOW, "raster capacity exceeded; oldest spikes kept");
   for (int q = 0; q < kNumPart; ++q) ctx->part_delta[q] = c1[q] - c0[q];
+  if (ctx->p.stats_on) {
+    nsx_status sf = stats_fold(ctx, ctx->t - n_steps, n_steps);
+    if (sf) return sf;
+  }
   if (st) {
     st->steps_done = n_steps;
     st->step_index = ctx->t;

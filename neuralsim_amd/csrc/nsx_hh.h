@@ -116,9 +116,10 @@ __device__ inline void hh_syn_store(const DevPtrs& s, uint32_t i, const HHSyn& y
 // arrivals a0 (NMDA through the Mg block at the step-start V, synapse/mod.rs:41-47), GABA-A on
 // inhibitory arrivals |a1|; the currents use the step-start V.  OU drive: the current value is
 // used and the process advanced by its exact one-step transition with the normal draw z.
+// lfp receives |I_AMPA| + |I_NMDA| + |I_GABA| (the neuron's term of the field-potential sum).
 template <int DRIVE>
 __device__ inline float hh_input(const StepParams& p, HHSyn& y, float v, float a0, float a1,
-                                 float z) {
+                                 float z, float& lfp) {
   float ga = y.ga * p.dec_e;
   float xa = a0 * p.gmax_ampa;
   ga = ga + xa;
@@ -137,8 +138,11 @@ __device__ inline float hh_input(const StepParams& p, HHSyn& y, float v, float a
   float de = p.e_exc - v, di = p.e_inh - v;
   float isyn = ga * de;
   float tn = gn * de;
-  isyn = isyn + tn;
   float tg = gg * di;
+  lfp = fabsf(isyn);
+  lfp = lfp + fabsf(tn);
+  lfp = lfp + fabsf(tg);
+  isyn = isyn + tn;
   isyn = isyn + tg;
   float iext = 0.0f;
   if (DRIVE == NSX_DRIVE_OU) {

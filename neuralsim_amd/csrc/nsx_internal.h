@@ -85,7 +85,12 @@ struct StepParams {
   uint64_t raster_cap;
   uint32_t batch;  // 1: neurons belong to the networks of a batched population
   uint32_t n_nets;
+  uint32_t stats_on;  // 1: per-step spike counts and the LFP partials are kept (nsx_set_stats)
+  uint32_t stat_cap;  // steps of the stat_counts ring
 };
+
+// per-step spike counts are added into this many stripes (one atomic per block and step)
+constexpr uint32_t kStatStripes = 16;
 
 // Device pointers used by the step kernels.
 struct DevPtrs {
@@ -103,6 +108,8 @@ struct DevPtrs {
   uint32_t *raster_step, *raster_id;
   unsigned long long* counters;  // see kCnt*
   uint32_t* step_counts;
+  uint32_t* stat_counts;  // [stat_cap][kStatStripes] spikes per step (stats on)
+  double* lfp_part;       // per-block running sum of |synaptic currents| (stats on)
   int32_t* tbase;
   // network
   const uint64_t* row_ptr;
@@ -206,12 +213,21 @@ struct nsx_ctx {
   nsx::Surrogate sur;
   // batched population
   std::vector<nsx::BatchNet> nets_h;
+  // SimulationStats kept on the host from the device's per-step counts (nsx_set_stats)
+  struct {
+    double rate = 0.0;         // low-pass mean firing rate (Hz)
+    uint64_t active = 0;       // largest per-step spike count
+    uint64_t last = 0;         // spikes of the last step
+    uint64_t steps = 0;        // steps folded
+  } hs;
 };
 
 // helpers shared by translation units
 namespace nsx {
 nsx_status set_error(nsx_ctx* ctx, nsx_status s, const std::string& msg);
 nsx_status hip_error(nsx_ctx* ctx, hipError_t e, const char* what);
+nsx_status stats_fold(nsx_ctx* ctx, uint64_t first_step, uint64_t n_steps);
+void destroy_graph(nsx_ctx* ctx);
 nsx_status dev_alloc(nsx_ctx* ctx, void** p, size_t bytes, bool zero);
 void dev_free(nsx_ctx* ctx, void* p);
 nsx_status build_csr_device(nsx_ctx* ctx, uint32_t* d_src, uint32_t* d_dst, float* d_w,

@@ -166,6 +166,23 @@ __device__ inline void spike_record(const StepParams& p, const DevPtrs& s, uint3
 
 // Block-local compaction of the spiking neurons of this 256-neuron block into its spike segment
 // (and the raster when recording).  Called by every thread of the block.
+// Adds the block's sum of x (one value per thread) to lfp_part[blk] in a fixed order (double).
+// Every thread of the block calls it.
+__device__ inline void lfp_accumulate(const DevPtrs& s, float x, uint32_t blk) {
+  __shared__ double red[kBlock / 64];
+  double v = (double)x;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBlock / 64; ++k) a += red[k];
+    s.lfp_part[blk] += a;
+  }
+}
+
 __device__ inline void block_compact(const StepParams& p, const DevPtrs& s, bool spiked,
                                      uint32_t i, int32_t t, uint32_t blk) {
   // Block-local compaction into this block's spike segment: wave ballots, wave counts in LDS,
@@ -189,6 +206,9 @@ __device__ inline void block_compact(const StepParams& p, const DevPtrs& s, bool
   if (threadIdx.x == 0) {
     s.bcnt[blk] = tot;
     if (tot) s.part[kPartSpikes * p.nblk + blk] += tot;
+    if (p.stats_on && tot)
+      atomicAdd(&s.stat_counts[((uint32_t)t % p.stat_cap) * kStatStripes + blk % kStatStripes],
+                tot);
   }
   if (p.raster_on && mask) {  // test/analysis path: append (step, id) pairs
     const int leader = __ffsll((long long)mask) - 1;

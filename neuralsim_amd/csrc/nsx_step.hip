@@ -66,6 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
   drive_stage<DRIVE>(p, s, t, s_drive);
   if (DRIVE != NSX_DRIVE_NONE) __syncthreads();  // the staged words
   bool spiked = false;
+  float lfp = 0.0f;  // HH: this neuron's |synaptic currents|
   if (live) {
     post_bump_prev<PAIR>(p, s, i, last, t);
     float a0, a1;
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       s.v[i] = v;
       s.u[i] = u;
     } else {  // Hodgkin-Huxley with AMPA / GABA-A / NMDA conductances
-      const float I = hh_input<DRIVE>(p, y, v, a0, a1, z);  // held over the sub-steps
+      const float I = hh_input<DRIVE>(p, y, v, a0, a1, z, lfp);  // held over the sub-steps
       hh_syn_store<DRIVE>(s, i, y);
       float m = w1, h = w2, nn = w3;
       float vn = v;
@@ -151,6 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
     }
     if (spiked) spike_record<PL>(p, s, i, t);
   }
+  if (MODEL == NSX_MODEL_HH && p.stats_on) lfp_accumulate(s, lfp, blockIdx.x);
   block_compact(p, s, spiked, i, t, blockIdx.x);
 }
 

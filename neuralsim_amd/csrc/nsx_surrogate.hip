@@ -424,13 +424,14 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
   drive_stage<DRIVE>(p, s, t, s_drive);
   __syncthreads();
   const MlpNorm& nm = S.norm;
+  float lfp = 0.0f;
   if (live) {
     post_bump_prev<PAIR>(p, s, i, last, t);
     float a0, a1;
     ring_take<true>(p, rg, a0, a1);
     float z = 0.0f;
     drive_draw<DRIVE>(p, s_drive, a0, z);
-    x[4] = hh_input<DRIVE>(p, y, x[0], a0, a1, z);
+    x[4] = hh_input<DRIVE>(p, y, x[0], a0, a1, z, lfp);
     hh_syn_store<DRIVE>(s, i, y);
   } else {
 #pragma unroll
@@ -447,6 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
     s.gn[i] = nx[3];
     if (spiked) spike_record<PAIR ? NSX_PLAST_PAIR : NSX_PLAST_NONE>(p, s, i, t);
   }
+  if (p.stats_on) lfp_accumulate(s, lfp, blockIdx.x);
   const unsigned long long fbm = __ballot(live && !in);
   if (lane == 0) s_fb[wid] = (uint32_t)__popcll(fbm);
   block_compact(p, s, spiked, i, t, blockIdx.x);  // synchronises the block

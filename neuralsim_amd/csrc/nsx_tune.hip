@@ -74,7 +74,10 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   const int32_t tb0 = ctx->tbase_host;
   const bool prof = ctx->profiling;
   const uint32_t split0 = ctx->p.prop_split, graph0 = ctx->graph_steps;
+  const uint32_t stats0 = ctx->p.stats_on;  // trials neither count nor fold statistics
   ctx->profiling = false;
+  ctx->p.stats_on = 0;
+  destroy_graph(ctx);
   nsx_tune_report r{};
   double best = 1e300;
   uint32_t best_split = split0, best_graph = graph0 ? graph0 : 50;
@@ -115,6 +118,8 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
       if ((s = trial(sp_best, g))) break;
   dev_free(ctx, snap);
   ctx->profiling = prof;
+  ctx->p.stats_on = stats0;
+  destroy_graph(ctx);
   if (s) {
     (void)nsx_set_prop_split(ctx, split0);
     (void)nsx_set_graph_steps(ctx, graph0);

@@ -25,6 +25,8 @@ def _c(a, dtype):
 class SimulationEngine:
     """Device-resident simulation of one network (SimulationEngine, engine.rs:43-80)."""
 
+    _stats_on = False
+
     def __init__(self, cfg):
         self._lib = load()
         self.cfg = cfg.copy()
@@ -208,10 +210,31 @@ class SimulationEngine:
         return out[: min(n.value, cap)]
 
     def stats(self):
-        """SimulationStats subset (engine.rs:21-33)."""
+        """SimulationStats (engine.rs:21-33): the full set while statistics are enabled
+        (set_stats), otherwise the running totals."""
+        if self._stats_on:
+            r = abi.NsxSimStats()
+            self._chk(self._lib.nsx_read_sim_stats(self._h, C.byref(r)), "nsx_read_sim_stats")
+            from .stats import SimulationStats
+            return SimulationStats.from_abi(r)
         st = self.step(0)
         return {"total_spikes": st.total_spikes, "total_events": st.total_events,
                 "sim_time_ms": st.sim_time_ms}
+
+    def set_stats(self, on=True):
+        """Keep SimulationStats on the device (per-step spike counts, LFP); clears them."""
+        self._chk(self._lib.nsx_set_stats(self._h, 1 if on else 0), "nsx_set_stats")
+        self._stats_on = bool(on)
+
+    def lfp(self):
+        """SimulationEngine::lfp (engine.rs:804-806); needs set_stats."""
+        return self.stats().lfp
+
+    def weight_stats(self):
+        """(mean, population std) of all synaptic weights (engine.rs:655-666)."""
+        m, s = C.c_double(), C.c_double()
+        self._chk(self._lib.nsx_weight_stats(self._h, C.byref(m), C.byref(s)), "nsx_weight_stats")
+        return m.value, s.value
 
     # ---- learned HH surrogate (no reference counterpart) ----------------------------------
     def surrogate_generate(self, n_samples, seed=1, stride=7, random_fraction=0.5,

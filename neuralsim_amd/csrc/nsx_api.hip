@@ -70,8 +70,8 @@ __global__ void k_init_state(StepParams p, DevPtrs s, int model, nsx_config cfg)
   if (i >= p.n) return;
   const NeuronId id = neuron_id(p, s, i, i / kBlock);
   s.last[i] = kNeverStep;
-  s.pre[i] = TraceRec{0.0f, kNeverStep};
-  s.post[i] = PostRec{0.0f, kNeverStep, kNeverStep, 0};
+  s.pre[i] = TraceRec{0.0f, kNeverStep, 0.0f, kNeverStep};
+  s.post[i] = PostRec{0.0f, kNeverStep, kNeverStep, 0u, {}};
   s.count[i] = 0u;
   s.refr[i] = 0;
   s.s0[i] = 0.0f;
@@ -395,7 +395,7 @@ nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out) {
   if ((s = dev_alloc(ctx, (void**)&d.refr, n * 4, true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.last, n * 4, true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.count, n * 4, true))) goto fail;
-  if ((s = dev_alloc(ctx, (void**)&d.pre, n * 8, true))) goto fail;
+  if ((s = dev_alloc(ctx, (void**)&d.pre, n * sizeof(TraceRec), true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.post, n * sizeof(PostRec), true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.ring, (size_t)ctx->D * ctx->ch * n * 8, true))) goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.spikes, (size_t)ctx->p.nblk * kBlock * 4, false))) goto fail;
@@ -489,12 +489,12 @@ static nsx_status put_traces(nsx_ctx* ctx, TraceRec* dst, const float* v, const 
   if (!v && !st) return NSX_OK;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<TraceRec> h(ctx->n);
-  NSX_HIP(ctx, hipMemcpy(h.data(), dst, ctx->n * 8, hipMemcpyDeviceToHost));
+  NSX_HIP(ctx, hipMemcpy(h.data(), dst, ctx->n * sizeof(TraceRec), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < ctx->n; ++i) {
     if (v) h[i].v = v[i];
     if (st) h[i].s = st[i];
   }
-  NSX_HIP(ctx, hipMemcpy(dst, h.data(), ctx->n * 8, hipMemcpyHostToDevice));
+  NSX_HIP(ctx, hipMemcpy(dst, h.data(), ctx->n * sizeof(TraceRec), hipMemcpyHostToDevice));
   return NSX_OK;
 }
 
@@ -502,7 +502,7 @@ static nsx_status get_traces(nsx_ctx* ctx, const TraceRec* src, float* v, int32_
   if (!v && !st) return NSX_OK;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<TraceRec> h(ctx->n);
-  NSX_HIP(ctx, hipMemcpy(h.data(), src, ctx->n * 8, hipMemcpyDeviceToHost));
+  NSX_HIP(ctx, hipMemcpy(h.data(), src, ctx->n * sizeof(TraceRec), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < ctx->n; ++i) {
     if (v) v[i] = h[i].v;
     if (st) st[i] = h[i].s;
@@ -520,6 +520,7 @@ static nsx_status put_post(nsx_ctx* ctx, const float* v, const int32_t* st, cons
     if (v) h[i].v = v[i];
     if (st) h[i].s = st[i];
     h[i].last = last ? last[i] : kNeverStep;
+    h[i].npend = 0;  // the uploaded weights carry every depression
   }
   NSX_HIP(ctx, hipMemcpy(ctx->d.post, h.data(), ctx->n * sizeof(PostRec), hipMemcpyHostToDevice));
   NSX_HIP(ctx, hipMemset(ctx->d.bcnt, 0, (size_t)ctx->p.nblk * 4));
@@ -604,6 +605,8 @@ nsx_status nsx_read_weights(nsx_ctx* ctx, float* weight, uint64_t m) {
   if (!ctx || !weight) return set_error(ctx, NSX_E_INVALID, "NULL argument");
   if (!ctx->has_network) return set_error(ctx, NSX_E_STATE, "no network loaded");
   if (m < ctx->m) return set_error(ctx, NSX_E_INVALID, "weight buffer too small");
+  nsx_status s = flush_pending(ctx, true);
+  if (s) return s;
   NSX_HIP(ctx, hipMemcpyAsync(weight, ctx->w, ctx->m * 4, hipMemcpyDeviceToHost, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return NSX_OK;
@@ -629,7 +632,12 @@ nsx_status nsx_get_step(nsx_ctx* ctx, uint64_t* step) {
 nsx_status nsx_set_step(nsx_ctx* ctx, uint64_t step) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (step >= (1ull << 30)) return set_error(ctx, NSX_E_INVALID, "step index limit");
-  // post trace bumps pending from the last step are keyed by that step: apply them now
+  // deferred depressions are applied, then the post trace bumps pending from the last step
+  // (keyed by that step)
+  if (ctx->has_network) {
+    nsx_status sf = flush_pending(ctx, true);
+    if (sf) return sf;
+  }
   if (ctx->cfg.plasticity == NSX_PLAST_PAIR && ctx->t > 0) {
     std::vector<float> v(ctx->n);
     std::vector<int32_t> st(ctx->n), last(ctx->n);

@@ -266,6 +266,7 @@ static void publish_network(nsx_ctx* ctx) {
   ctx->d.csc_slot = ctx->csc_slot;
   ctx->d.csc_src = ctx->csc_src;
   ctx->has_network = true;
+  (void)flush_pending(ctx, false);  // depressions pending on the old weights are dropped
   // a captured graph holds the old pointers
   if (ctx->graph_exec) {
     (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -563,6 +564,10 @@ nsx_status nsx_read_csr(nsx_ctx* ctx, uint64_t* row_ptr, uint32_t* col, float* w
                         uint8_t* delay) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (!ctx->has_network) return set_error(ctx, NSX_E_STATE, "no network loaded");
+  if (weight) {
+    nsx_status s = flush_pending(ctx, true);
+    if (s) return s;
+  }
   if (row_ptr)
     NSX_HIP(ctx, hipMemcpyAsync(row_ptr, ctx->row_ptr, (ctx->n + 1) * 8, hipMemcpyDeviceToHost,
                                 ctx->stream));

@@ -14,23 +14,36 @@ namespace nsx {
 constexpr int kMaxPoisson = 32;
 constexpr uint32_t kBlock = 256;
 
-// Lazily decayed STDP trace of one neuron: value right after its last bump and that step.
-struct __attribute__((aligned(8))) TraceRec {
+// Lazily decayed STDP trace of one neuron: value right after its last bump and that step, and
+// the same pair before that bump (pair STDP: the trace between the last two spikes).
+struct __attribute__((aligned(16))) TraceRec {
   float v;
   int32_t s;
+  float v_prev;
+  int32_t s_prev;
 };
 
-// Post-synaptic record: the lazily decayed trace {v, s}, plus the step of the neuron's last spike
-// so that the potentiation pass sees same-step post spikes in the same cache line.  The trace
-// bump of a spike at step t is applied by the update kernel of step t+1 (after every pass of
-// step t has read the pre-bump value).  Triplet STDP keeps no trace: s holds the step of the
+// Pair STDP defers the depression of a post spike: up to kPostPend post spikes of a neuron are
+// kept in its record, and each is applied to an in-synapse either when the synapse's source
+// next fires (the propagation pass reads the row's weights anyway) or when the list fills, by
+// one walk over the neuron's in-synapses for all of them.  Every depression is applied once, in
+// time order with the synapse's potentiations, so the weights equal the per-spike updates.
+constexpr uint32_t kPostPend = 12;
+
+// Post-synaptic record, one 64-byte line: the lazily decayed trace {v, s}, the step of the
+// neuron's last spike (same-step post spikes seen by the potentiation pass), and the pending
+// post spikes.  The trace bump of a spike at step t is applied by the update kernel of step t+1
+// (after every pass of step t has read the pre-bump value), which also empties a list that the
+// synapse pass of step t flushed.  Triplet STDP keeps no trace: s holds the step of the
 // neuron's spike before `last`.
-struct __attribute__((aligned(16))) PostRec {
+struct __attribute__((aligned(64))) PostRec {
   float v;
   int32_t s;
   int32_t last;
-  int32_t pad;
+  uint32_t npend;
+  int32_t pend[kPostPend];
 };
+static_assert(sizeof(PostRec) == 64 && sizeof(TraceRec) == 16, "record layouts");
 
 // Hodgkin-Huxley constants of one Euler sub-step.
 struct HHConst {
@@ -228,6 +241,8 @@ nsx_status set_error(nsx_ctx* ctx, nsx_status s, const std::string& msg);
 nsx_status hip_error(nsx_ctx* ctx, hipError_t e, const char* what);
 nsx_status stats_fold(nsx_ctx* ctx, uint64_t first_step, uint64_t n_steps);
 void destroy_graph(nsx_ctx* ctx);
+// Pair STDP: apply (apply = true) or drop every pending deferred depression and empty the lists.
+nsx_status flush_pending(nsx_ctx* ctx, bool apply);
 nsx_status dev_alloc(nsx_ctx* ctx, void** p, size_t bytes, bool zero);
 void dev_free(nsx_ctx* ctx, void* p);
 nsx_status build_csr_device(nsx_ctx* ctx, uint32_t* d_src, uint32_t* d_dst, float* d_w,

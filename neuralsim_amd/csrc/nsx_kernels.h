@@ -23,15 +23,7 @@ __device__ inline TraceRec trace_bump(const TraceRec rec, const float* tab, int3
   const float q = trace_at(tab, tab_n, t - rec.s);
   float x = rec.v * q;
   x = x * q;
-  return TraceRec{x + 1.0f, t};
-}
-
-__device__ inline PostRec post_bump(const PostRec rec, const float* tab, int32_t tab_n,
-                                    int32_t t) {
-  const float q = trace_at(tab, tab_n, t - rec.s);
-  float x = rec.v * q;
-  x = x * q;
-  return PostRec{x + 1.0f, t, rec.last, 0};
+  return TraceRec{x + 1.0f, t, rec.v, rec.s};
 }
 
 // Synapse k of target j of a fixed in-degree network (n neurons, [0, n_exc) excitatory): source
@@ -108,11 +100,23 @@ __device__ inline void drive_draw(const StepParams& p, const uint32_t* w, float&
 // neuron's last spike step as loaded at the start of the update (not loaded, and kNeverStep, in a
 // block whose spike segment of step t-1 is empty); each thread bumps its own neuron, before any
 // spike_record of step t.
+// A list of pending depressions that filled at step t-1 was applied by that step's synapse pass
+// and is emptied here.
 template <bool PAIR>
 __device__ inline void post_bump_prev(const StepParams& p, const DevPtrs& s, uint32_t i,
                                       int32_t last, int32_t t) {
-  if (PAIR && last == t - 1 && i < p.n_exc)
-    s.post[i] = post_bump(s.post[i], s.mtab, p.mtab_n, t - 1);
+  if (PAIR && last == t - 1 && i < p.n_exc) {
+    PostRec& r = s.post[i];
+    const float v = r.v;
+    const int32_t st = r.s;
+    const uint32_t np = r.npend;
+    const float q = trace_at(s.mtab, p.mtab_n, (t - 1) - st);
+    float x = v * q;
+    x = x * q;
+    r.v = x + 1.0f;
+    r.s = t - 1;
+    if (np == kPostPend) r.npend = 0;
+  }
 }
 
 // Neuron i's ring slot of step t: ring_load reads channel 0 (and channel 1 when the network has
@@ -154,7 +158,13 @@ __device__ inline void spike_record(const StepParams& p, const DevPtrs& s, uint3
   s.last[i] = t;
   if (PL == NSX_PLAST_PAIR && i < p.n_exc) {
     s.pre[i] = trace_bump(s.pre[i], s.ptab, p.ptab_n, t);
-    s.post[i].last = t;
+    PostRec& r = s.post[i];
+    r.last = t;
+    const uint32_t np = r.npend;  // < kPostPend: a full list was emptied by post_bump_prev
+    if (np < kPostPend) {
+      r.pend[np] = t;  // this spike's depression is pending
+      r.npend = np + 1;
+    }
   }
   if (PL == NSX_PLAST_TRIPLET && i < p.n_exc) {
     PostRec r = s.post[i];

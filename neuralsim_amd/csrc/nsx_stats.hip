@@ -104,9 +104,10 @@ nsx_status nsx_weight_stats(nsx_ctx* ctx, double* mean, double* stddev) {
   *mean = 0.0;
   *stddev = 0.0;
   if (ctx->m == 0) return NSX_OK;  // the reference leaves both at 0 without synapses
-  double* d = nullptr;
-  nsx_status s = dev_alloc(ctx, (void**)&d, kWBlocks * 8, false);
+  nsx_status s = flush_pending(ctx, true);
   if (s) return s;
+  double* d = nullptr;
+  if ((s = dev_alloc(ctx, (void**)&d, kWBlocks * 8, false))) return s;
   std::vector<double> h(kWBlocks);
   double mu = 0.0;
   for (int pass = 0; pass < 2; ++pass) {

@@ -195,9 +195,64 @@ __device__ inline uint64_t chunk_slot(const RowChunk& rc, uint32_t f, uint32_t& 
   return rc.beg[r] + (f - excl);
 }
 
+// Pair STDP on an E->E synapse whose source fired at step t (a: the source's record, bumped at
+// t; pr: the target's).  pair_pending: the target's pending depressions from post spikes after
+// the source's previous spike (their pre trace is the one between the source's last two
+// spikes), in time order -- the weight the spike delivers.  pair_at_pre: then potentiation by
+// the post trace, then depression if the target fired at t as well.
+__device__ inline float pair_pending(const StepParams& p, const DevPtrs& s, float w,
+                                     const PostRec& pr, const TraceRec& a, int32_t t) {
+#pragma unroll
+  for (uint32_t q = 0; q < kPostPend; ++q) {
+    const int32_t sp = pr.pend[q];
+    if (q < pr.npend && sp > a.s_prev && sp < t) {
+      const float x = trace_now(a.v_prev, a.s_prev, s.ptab, p.ptab_n, sp);
+      const float dlt = p.a_minus * x;
+      w = w - dlt;
+      w = clampf_(w, p.w_min, p.w_max);
+    }
+  }
+  return w;
+}
+
+__device__ inline float pair_at_pre(const StepParams& p, const DevPtrs& s, float w,
+                                    const PostRec& pr, const TraceRec& a, int32_t t) {
+  // the source fired at t, so its pre trace is the record bumped at t (factor P[0] = 1)
+  const float x = trace_now(pr.v, pr.s, s.mtab, p.mtab_n, t);
+  const float dlt = p.a_plus * x;
+  w = w + dlt;
+  w = clampf_(w, p.w_min, p.w_max);
+  if (pr.last == t) {  // target fired at t as well: depression follows
+    const float y = p.a_minus * a.v;
+    w = w - y;
+    w = clampf_(w, p.w_min, p.w_max);
+  }
+  return w;
+}
+
+// Deferred pair depressions of post neuron j's pending spikes pend[0, np) on one in-synapse whose
+// source has record a: a spike at step sp applies when the source has not fired since (a.s <
+// sp; otherwise the source's propagation applied it, or owned it as a same-step pair).
+__device__ inline float pair_flush(const StepParams& p, const DevPtrs& s, float w,
+                                   const int32_t* pend, uint32_t np, const TraceRec& a,
+                                   bool& changed) {
+#pragma unroll
+  for (uint32_t q = 0; q < kPostPend; ++q) {
+    const int32_t sp = pend[q];
+    if (q < np && a.s < sp) {
+      const float x = trace_now(a.v, a.s, s.ptab, p.ptab_n, sp);
+      const float dlt = p.a_minus * x;
+      w = w - dlt;
+      w = clampf_(w, p.w_min, p.w_max);
+      changed = true;
+    }
+  }
+  return w;
+}
+
 // Propagation work item: part q of neuron block b.  PL: STDP applied on the spiking rows' E->E
-// slots -- pair: potentiation by the post trace (then depression if the target fired at t);
-// triplet: depression by the target's last spike.
+// slots -- pair: the target's pending depressions, potentiation by the post trace (then
+// depression if the target fired at t); triplet: depression by the target's last spike.
 template <int PL>
 __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
                                uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane) {
@@ -228,13 +283,24 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
       const uint64_t k1 = two ? chunk_slot(rc, f + 64, r1) : k0;
       const uint32_t src1 = rc.src[r1];
       const uint32_t j0 = s.col[k0], j1 = s.col[k1];
-      const float w0 = s.w[k0], w1 = s.w[k1];
+      float w0 = s.w[k0], w1 = s.w[k1];
       const uint32_t d0 = s.delay[k0], d1 = s.delay[k1];
       const bool pot0 = PL != NSX_PLAST_NONE && src0 < p.n_exc && j0 < p.n_exc;
       const bool pot1 = PL != NSX_PLAST_NONE && two && src1 < p.n_exc && j1 < p.n_exc;
-      PostRec pr0{0.0f, kNeverStep, kNeverStep, 0}, pr1{0.0f, kNeverStep, kNeverStep, 0};
-      if (pot0) pr0 = s.post[j0];
+      PostRec pr0{0.0f, kNeverStep, kNeverStep, 0u, {}}, pr1{0.0f, kNeverStep, kNeverStep, 0u, {}};
+      if (pot0) pr0 = s.post[j0];  // one 64-byte line: trace, last spike, pending spikes
       if (pot1) pr1 = s.post[j1];
+      TraceRec a0{}, a1{};
+      if (PL == NSX_PLAST_PAIR) {  // the target's pending depressions precede the delivery
+        if (pot0) {
+          a0 = s.pre[src0];
+          w0 = pair_pending(p, s, w0, pr0, a0, t);
+        }
+        if (pot1) {
+          a1 = s.pre[src1];
+          w1 = pair_pending(p, s, w1, pr1, a1, t);
+        }
+      }
       uint32_t sl0 = tm + d0, sl1 = tm + d1;
       sl0 = sl0 >= p.D ? sl0 - p.D : sl0;
       sl1 = sl1 >= p.D ? sl1 - p.D : sl1;
@@ -263,31 +329,12 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
         }
       }
       if (PL == NSX_PLAST_PAIR && (pot0 || pot1)) {
-        // the source fired at t, so its pre trace is the record bumped at t (factor P[0] = 1)
         if (pot0) {
-          const float x = trace_now(pr0.v, pr0.s, s.mtab, p.mtab_n, t);
-          const float dlt = p.a_plus * x;
-          float wn = w0 + dlt;
-          wn = clampf_(wn, p.w_min, p.w_max);
-          if (pr0.last == t) {  // target fired at t as well: depression follows
-            const float y = p.a_minus * s.pre[src0].v;
-            wn = wn - y;
-            wn = clampf_(wn, p.w_min, p.w_max);
-          }
-          s.w[k0] = wn;
+          s.w[k0] = pair_at_pre(p, s, w0, pr0, a0, t);
           ltp += 1;
         }
         if (pot1) {
-          const float x = trace_now(pr1.v, pr1.s, s.mtab, p.mtab_n, t);
-          const float dlt = p.a_plus * x;
-          float wn = w1 + dlt;
-          wn = clampf_(wn, p.w_min, p.w_max);
-          if (pr1.last == t) {
-            const float y = p.a_minus * s.pre[src1].v;
-            wn = wn - y;
-            wn = clampf_(wn, p.w_min, p.w_max);
-          }
-          s.w[k1] = wn;
+          s.w[k1] = pair_at_pre(p, s, w1, pr1, a1, t);
           ltp += 1;
         }
       }
@@ -312,8 +359,9 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
 }
 
 // Post-spike work item: part q of neuron block b (excitatory neurons only), over the E->E
-// in-slots of its spiking neurons whose source did not fire at t -- pair: depression by the pre
-// trace; triplet: potentiation by the source's last spike and the target's previous spike.
+// in-slots of its spiking neurons -- pair: only neurons whose list of pending depressions
+// filled at t, all of them applied (pair_flush); triplet: potentiation by the source's last
+// spike and the target's previous spike, for sources that did not fire at t.
 template <int PL>
 __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
                              uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane) {
@@ -327,7 +375,7 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
     uint64_t beg = 0;
     if (lane < nr) {
       j = s.spikes[b * kBlock + q + (c0 + lane) * P];
-      if (j < p.n_exc) {
+      if (j < p.n_exc && (PL != NSX_PLAST_PAIR || s.post[j].npend == kPostPend)) {
         beg = s.csc_ptr[j];
         deg = (uint32_t)(s.csc_ptr[j + 1] - beg);
       }
@@ -374,23 +422,25 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
         r = r1;
         continue;
       }
+      // pair: the row's post neuron has a full list of pending depressions (one line)
+      const PostRec& pj0 = s.post[rc.src[r]];
+      const PostRec& pj1 = s.post[rc.src[r1]];
       const TraceRec a0 = s.pre[i0], a1 = s.pre[i1];
-      // a source that fired at t owns this slot in its propagation item
-      const bool own0 = a0.s != t, own1 = two && a1.s != t;
+      // a source that fired at or after every pending spike has none left on this slot
+      const bool own0 = a0.s < pj0.pend[kPostPend - 1];
+      const bool own1 = two && a1.s < pj1.pend[kPostPend - 1];
       float w0 = 0.0f, w1 = 0.0f;
       if (own0) w0 = s.w[k0];
       if (own1) w1 = s.w[k1];
       if (own0) {
-        const float x = trace_now(a0.v, a0.s, s.ptab, p.ptab_n, t);
-        const float dlt = p.a_minus * x;
-        const float wn = w0 - dlt;
-        s.w[k0] = clampf_(wn, p.w_min, p.w_max);
+        bool ch = false;
+        const float wn = pair_flush(p, s, w0, pj0.pend, kPostPend, a0, ch);
+        if (ch) s.w[k0] = wn;
       }
       if (own1) {
-        const float x = trace_now(a1.v, a1.s, s.ptab, p.ptab_n, t);
-        const float dlt = p.a_minus * x;
-        const float wn = w1 - dlt;
-        s.w[k1] = clampf_(wn, p.w_min, p.w_max);
+        bool ch = false;
+        const float wn = pair_flush(p, s, w1, pj1.pend, kPostPend, a1, ch);
+        if (ch) s.w[k1] = wn;
       }
       r = r1;
     }
@@ -530,6 +580,42 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
 hipError_t enqueue_advance(nsx_ctx* ctx, int32_t g) {
   hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, ctx->stream, ctx->d.tbase, g);
   return hipGetLastError();
+}
+
+// Every excitatory neuron's pending depressions applied to its E->E in-synapses (apply) or
+// dropped, and its list emptied.  A full list was applied by the synapse pass of the step it
+// filled in and is only emptied.
+__global__ void k_pend_flush(StepParams p, DevPtrs s, int apply) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < p.n_exc;
+       j += gridDim.x * blockDim.x) {
+    PostRec& r = s.post[j];
+    const uint32_t np = r.npend;
+    if (np == 0) continue;
+    if (apply && np < kPostPend) {
+      int32_t pend[kPostPend];
+#pragma unroll
+      for (uint32_t q = 0; q < kPostPend; ++q) pend[q] = r.pend[q];
+      for (uint64_t e = s.csc_ptr[j]; e < s.csc_ptr[j + 1]; ++e) {
+        const TraceRec a = s.pre[s.csc_src[e]];
+        const uint64_t k = s.csc_slot[e];
+        bool ch = false;
+        const float w = pair_flush(p, s, s.w[k], pend, np, a, ch);
+        if (ch) s.w[k] = w;
+      }
+    }
+    r.npend = 0;
+  }
+}
+
+nsx_status flush_pending(nsx_ctx* ctx, bool apply) {
+  if (ctx->cfg.plasticity != NSX_PLAST_PAIR || !ctx->csc_ptr || ctx->p.n_exc == 0)
+    return NSX_OK;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((ctx->p.n_exc + 255) / 256, 1u << 14);
+  hipLaunchKernelGGL(k_pend_flush, dim3(blocks), dim3(256), 0, ctx->stream, ctx->p, ctx->d,
+                     apply ? 1 : 0);
+  NSX_HIP(ctx, hipGetLastError());
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return NSX_OK;
 }
 
 }  // namespace nsx

@@ -27,7 +27,7 @@ static std::vector<Buf> mutable_buffers(nsx_ctx* ctx) {
       {d.v, n * 4},       {d.u, n * 4},       {d.gm, n * 4},        {d.gh, n * 4},
       {d.gn, n * 4},      {d.s0, n * 4},      {d.s1, n * 4},        {d.s2, n * 4},
       {d.drv, n * 4},     {d.refr, n * 4},    {d.last, n * 4},      {d.count, n * 4},
-      {d.pre, n * 8},     {d.post, n * sizeof(PostRec)},
+      {d.pre, n * sizeof(TraceRec)}, {d.post, n * sizeof(PostRec)},
       {d.ring, (size_t)ctx->D * ctx->ch * n * 8},
       {d.spikes, nblk * kBlock * 4},          {d.bcnt, nblk * 4},
       {d.part, nblk * kNumPart * 8},          {d.counters, kNumCounters * 8},

@@ -288,7 +288,7 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
       const bool pot0 = PL != NSX_PLAST_NONE && src0 < p.n_exc && j0 < p.n_exc;
       const bool pot1 = PL != NSX_PLAST_NONE && two && src1 < p.n_exc && j1 < p.n_exc;
       PostRec pr0{0.0f, kNeverStep, kNeverStep, 0u, {}}, pr1{0.0f, kNeverStep, kNeverStep, 0u, {}};
-      if (pot0) pr0 = s.post[j0];  // one 64-byte line: trace, last spike, pending spikes
+      if (pot0) pr0 = s.post[j0];  // 32 bytes: trace, last spike, pending spikes
       if (pot1) pr1 = s.post[j1];
       TraceRec a0{}, a1{};
       if (PL == NSX_PLAST_PAIR) {  // the target's pending depressions precede the delivery
@@ -422,7 +422,7 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
         r = r1;
         continue;
       }
-      // pair: the row's post neuron has a full list of pending depressions (one line)
+      // pair: the row's post neuron has a full list of pending depressions
       const PostRec& pj0 = s.post[rc.src[r]];
       const PostRec& pj1 = s.post[rc.src[r1]];
       const TraceRec a0 = s.pre[i0], a1 = s.pre[i1];

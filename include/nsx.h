@@ -220,6 +220,23 @@ nsx_status nsx_set_prop_split(nsx_ctx* ctx, uint32_t waves_per_block);
 /* Use a captured hipGraph of `steps_per_graph` steps for nsx_step (0 = direct launches). */
 nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph);
 
+/* How propagated events reach their ring slot (the reference adds each event into
+ * pending_input, engine.rs:414-494).  Results are bit-identical in both modes: the arrivals are
+ * int64 sums, whatever their order.
+ * ATOMIC: one 64-bit device atomic per event into ring[(t+d) mod D][channel][target].
+ * BINNED: the synapse pass writes each event into a bucket of (delay, target tile) of its
+ *   step's buffer (two counting passes give each workgroup its offsets); at arrival one
+ *   workgroup per tile sums its buckets in LDS and adds them into the ring slot.  Needs
+ *   (D - 1) * tiles <= 8192 (tiles = ceil(N * channels / tile_entries)) and M < 2^32; a step
+ *   with more events than cap_events (0: min(M, 2^24, 2^31 / (8 D)), at least min(M, 2^16))
+ *   falls back to atomics.  Costs D * cap_events * 8 bytes. */
+#define NSX_DELIVERY_ATOMIC 0u
+#define NSX_DELIVERY_BINNED 1u
+/* tile_entries: 4096, 8192 or 16384 LDS accumulators per tile (0: automatic). */
+nsx_status nsx_set_delivery(nsx_ctx* ctx, uint32_t mode, uint64_t cap_events,
+                            uint32_t tile_entries);
+nsx_status nsx_get_delivery(nsx_ctx* ctx, uint32_t* mode);
+
 /* Raster (record_raster=1): (step, neuron) pairs in step order; within a step, neuron order.
  * *n_out = spikes available; at most cap are copied.  clear=1 empties the raster. */
 nsx_status nsx_read_raster(nsx_ctx* ctx, uint32_t* steps, uint32_t* neurons, uint64_t cap,
@@ -259,8 +276,8 @@ nsx_status nsx_weight_stats(nsx_ctx* ctx, double* mean, double* stddev);
 /* Kernel timing: when enabled, every launch of the named kernel class in nsx_step is bracketed
  * by hipEvents on the stream it runs on (steps run as direct launches, not graphs).  Enabling
  * starts a fresh measurement; disabling keeps it readable.  kind: 0 neuron update, 1 synapse
- * pass (propagation and STDP); bytes report the STDP slots of the last call (1: at pre spikes,
- * 2: at post spikes). */
+ * pass (propagation and STDP), 2 binned delivery's counting passes, 3 its arrival pass; bytes
+ * report the STDP slots of the last call (1: at pre spikes, 2: at post spikes). */
 nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable);
 nsx_status nsx_kernel_time(nsx_ctx* ctx, int kind, double* total_ms, uint64_t* launches,
                            uint64_t* bytes);
@@ -303,14 +320,17 @@ nsx_status nsx_backend_step_spmv(nsx_ctx* ctx, float* input_current);
  * No reference counterpart (the reference has one fixed wgpu dispatch shape, gpu_backend.rs:
  * 921-981).  Times trial_steps steps for each candidate propagation split (wavefronts sharing a
  * neuron block's spike segment: 1, 2, 4, 8, 16) at a 50-step graph window, then each graph window
- * (10, 25, 100, 200 steps) at the best split, every trial from a device-side snapshot of the
- * mutable state; restores the state and the step index, and sets the fastest pair. */
+ * (10, 25, 100, 200 steps) at the best split, then binned delivery with 8192- and 16384-entry
+ * tiles where the network allows it, every trial from a device-side snapshot of the mutable
+ * state; restores the state and the step index, and sets the fastest combination. */
 typedef struct nsx_tune_report {
   uint32_t prop_split, graph_steps; /* chosen                                                 */
   uint32_t n_trials, _pad;
   double best_us_per_step;
   double us_per_step[32];           /* per trial, in trial order                              */
   uint32_t trial_split[32], trial_graph[32];
+  uint32_t delivery, tile_entries;  /* chosen (tile_entries 0 with atomic delivery)           */
+  uint32_t trial_tile[32];          /* per trial: 0 atomic delivery, else binned tile entries */
 } nsx_tune_report;
 
 nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* report);

@@ -33,6 +33,9 @@ DRIVE_OU = 3
 PLAST_NONE = 0
 PLAST_PAIR = 1
 PLAST_TRIPLET = 2
+
+DELIVERY_ATOMIC = 0  # nsx_set_delivery: one 64-bit ring atomic per event
+DELIVERY_BINNED = 1  # events bucketed by (delay, target tile), summed in LDS at arrival
 # nsx_hh_spike_mode
 HH_SPIKE_CROSSING = 0
 HH_SPIKE_REFERENCE_RESET = 1
@@ -137,6 +140,9 @@ class NsxTuneReport(C.Structure):
         ("us_per_step", C.c_double * 32),
         ("trial_split", C.c_uint32 * 32),
         ("trial_graph", C.c_uint32 * 32),
+        ("delivery", C.c_uint32),
+        ("tile_entries", C.c_uint32),
+        ("trial_tile", C.c_uint32 * 32),
     ]
 
 

@@ -467,7 +467,7 @@ nsx_status nsx_reset_state(nsx_ctx* ctx) {
   ctx->hs = {};
   ctx->t = 0;
   ctx->tbase_host = 0;
-  return NSX_OK;
+  return bin_clear(ctx);
 }
 
 nsx_status nsx_network_size(nsx_ctx* ctx, uint64_t* n, uint64_t* m) {
@@ -577,7 +577,7 @@ nsx_status nsx_upload_state(nsx_ctx* ctx, const nsx_state_view* sv) {
       (s = copy_f(ctx, d.ring, sv->ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)))
     return s;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return NSX_OK;
+  return bin_clear(ctx);  // the uploaded ring holds every pending input
 }
 
 nsx_status nsx_read_state(nsx_ctx* ctx, nsx_state_view* sv) {
@@ -585,7 +585,8 @@ nsx_status nsx_read_state(nsx_ctx* ctx, nsx_state_view* sv) {
   const size_t b = ctx->n * 4;
   const hipMemcpyKind k = hipMemcpyDeviceToHost;
   DevPtrs& d = ctx->d;
-  nsx_status s;
+  nsx_status s = bin_materialize(ctx);  // binned events in flight join the ring
+  if (s) return s;
   if ((s = copy_f(ctx, sv->v, d.v, b, k)) || (s = copy_f(ctx, sv->u, d.u, b, k)) ||
       (s = copy_f(ctx, sv->m, d.gm, b, k)) || (s = copy_f(ctx, sv->h, d.gh, b, k)) ||
       (s = copy_f(ctx, sv->n, d.gn, b, k)) || (s = copy_f(ctx, sv->syn0, d.s0, b, k)) ||
@@ -636,6 +637,7 @@ nsx_status nsx_set_step(nsx_ctx* ctx, uint64_t step) {
   // (keyed by that step)
   if (ctx->has_network) {
     nsx_status sf = flush_pending(ctx, true);
+    if (!sf) sf = bin_materialize(ctx);  // the ring then holds every pending input
     if (sf) return sf;
   }
   if (ctx->cfg.plasticity == NSX_PLAST_PAIR && ctx->t > 0) {
@@ -673,7 +675,7 @@ nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   // switching on starts a fresh measurement; switching off keeps it for nsx_kernel_time
   if (enable && !ctx->profiling)
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       ctx->kt_ms[k] = 0;
       ctx->kt_launches[k] = 0;
     }
@@ -683,7 +685,7 @@ nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable) {
 
 nsx_status nsx_kernel_time(nsx_ctx* ctx, int kind, double* total_ms, uint64_t* launches,
                            uint64_t* bytes) {
-  if (!ctx || kind < 0 || kind > 2) return set_error(ctx, NSX_E_INVALID, "bad argument");
+  if (!ctx || kind < 0 || kind > 3) return set_error(ctx, NSX_E_INVALID, "bad argument");
   if (total_ms) *total_ms = ctx->kt_ms[kind];
   if (launches) *launches = ctx->kt_launches[kind];
   // slot counts of the last nsx_step call: kind 1 -> potentiated slots, kind 2 -> depressed
@@ -730,6 +732,10 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   if (ctx->t + n_steps >= (1ull << 30)) return set_error(ctx, NSX_E_INVALID, "step index limit");
   if (ctx->cfg.model == NSX_MODEL_HH_SURROGATE && !(ctx->sur.has_params && ctx->sur.has_norm))
     return set_error(ctx, NSX_E_STATE, "surrogate parameters not set (train or load them)");
+  if (ctx->bin_dirty) {  // binned delivery sized for the network now loaded
+    nsx_status sb = bin_configure(ctx, ctx->delivery, ctx->bin_cap_req, ctx->bin_te_req);
+    if (sb) return sb;
+  }
   if (ctx->p.stats_on && n_steps > ctx->p.stat_cap) {
     // the per-step count ring holds stat_cap steps: run in chunks, folding after each
     nsx_step_stats one{};
@@ -764,9 +770,7 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   NSX_HIP(ctx, hipEventCreate(&e_end));
   NSX_HIP(ctx, hipEventRecord(e_beg, ctx->stream));
   const bool prof = ctx->profiling;
-  const bool pair = ctx->cfg.plasticity == NSX_PLAST_PAIR;
-  const size_t nev = 3;  // events around k_update and k_synapses
-  (void)pair;
+  const size_t nev = 5;  // events around the update, counting, synapse and arrival launches
   if (prof && ctx->ev_pool.size() < nev * n_steps) {
     size_t have = ctx->ev_pool.size();
     ctx->ev_pool.resize(nev * n_steps);
@@ -818,11 +822,14 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   if (prof) {
     for (uint64_t k = first_direct; k < n_steps; ++k) {
       hipEvent_t* evs = &ctx->ev_pool[nev * k];
+      static const int kind_of[4] = {0, 2, 1, 3};  // event interval -> nsx_kernel_time kind
       for (size_t q = 0; q + 1 < nev; ++q) {
+        const int kd = kind_of[q];
+        if ((kd == 2 || kd == 3) && !ctx->p.bin_on) continue;
         float ms = 0.0f;
         NSX_HIP(ctx, hipEventElapsedTime(&ms, evs[q], evs[q + 1]));
-        ctx->kt_ms[q] += ms;
-        ctx->kt_launches[q] += 1;
+        ctx->kt_ms[kd] += ms;
+        ctx->kt_launches[kd] += 1;
       }
     }
   }

@@ -267,6 +267,11 @@ static void publish_network(nsx_ctx* ctx) {
   ctx->d.csc_src = ctx->csc_src;
   ctx->has_network = true;
   (void)flush_pending(ctx, false);  // depressions pending on the old weights are dropped
+  // binned delivery: events in flight join the ring; buffers are re-sized before the next step
+  if (ctx->delivery == NSX_DELIVERY_BINNED) {
+    (void)bin_materialize(ctx);
+    ctx->bin_dirty = true;
+  }
   // a captured graph holds the old pointers
   if (ctx->graph_exec) {
     (void)hipGraphExecDestroy(ctx->graph_exec);

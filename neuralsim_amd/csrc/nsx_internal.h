@@ -14,6 +14,13 @@ namespace nsx {
 constexpr int kMaxPoisson = 32;
 constexpr uint32_t kBlock = 256;
 
+// Binned delivery (nsx_set_delivery): the synapse pass writes each event into a bucket of
+// (delay, target tile) instead of a ring atomic; a tile's buckets are summed in LDS at arrival.
+constexpr uint32_t kBinThreads = 1024;    // workgroup of the binning / synapse kernels
+constexpr uint32_t kTileEntries = 16384;  // int64 LDS accumulators of one tile (128 KB)
+constexpr uint32_t kMaxBins = 8192;       // (D - 1) * tiles
+constexpr uint32_t kMaxArriveSegs = 256;  // D - 1 (ring_slots is at most 256)
+
 // Lazily decayed STDP trace of one neuron: value right after its last bump and that step, and
 // the same pair before that bump (pair STDP: the trace between the last two spikes).
 struct __attribute__((aligned(16))) TraceRec {
@@ -102,6 +109,9 @@ struct StepParams {
   uint32_t n_nets;
   uint32_t stats_on;  // 1: per-step spike counts and the LFP partials are kept (nsx_set_stats)
   uint32_t stat_cap;  // steps of the stat_counts ring
+  // binned delivery (bin_on = 1): tiles of 2^bin_tshift neurons, bucket (d - 1) * ntiles + tile
+  uint32_t bin_on, bin_tshift, bin_ntiles, bin_nbins, bin_groups;
+  uint32_t bin_cap;  // events one step's bucket buffer holds; more fall back to ring atomics
 };
 
 // per-step spike counts are added into this many stripes (one atomic per block and step)
@@ -137,6 +147,11 @@ struct DevPtrs {
   // batched population
   const uint32_t* blk_net;  // network of each neuron block
   const BatchNet* bnet;
+  // binned delivery
+  uint2* bin_ev;       // [D][bin_cap] events {LDS entry, weight bits} of the step t mod D
+  uint32_t* bin_tab;   // [D][nbins + 1] bucket starts in that step's buffer (all 0: none)
+  uint32_t* bin_hist;  // [groups][nbins] events per group and bucket, then group offsets
+  uint32_t* bin_tot;   // [nbins] events per bucket
 };
 
 // Identity of a neuron for its RNG streams and E/I type: the network's seed and local index in
@@ -210,9 +225,9 @@ struct nsx_ctx {
   hipGraph_t graph = nullptr;
   // profiling
   bool profiling = false;
-  double kt_ms[3] = {0, 0, 0};
-  uint64_t kt_launches[3] = {0, 0, 0};
-  std::vector<hipEvent_t> ev_pool;  // direct launches: 3 events per step
+  double kt_ms[4] = {0, 0, 0, 0};
+  uint64_t kt_launches[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> ev_pool;  // direct launches: 5 events per step
   uint32_t last_call_steps = 0;
   std::vector<unsigned long long> part_host;
   uint64_t part_delta[4] = {0, 0, 0, 0};
@@ -228,6 +243,12 @@ struct nsx_ctx {
   nsx::Surrogate sur;
   // batched population
   std::vector<nsx::BatchNet> nets_h;
+  // binned delivery: requested mode, capacity override, buffers sized for (n, m, D)
+  uint32_t delivery = NSX_DELIVERY_ATOMIC;
+  uint64_t bin_cap_req = 0;
+  uint32_t bin_te_req = 0;  // tile entries requested (0: automatic)
+  uint32_t bin_te = 0;      // tile entries in use
+  bool bin_dirty = false;  // the network changed: buffers are re-sized before the next step
   // SimulationStats kept on the host from the device's per-step counts (nsx_set_stats)
   struct {
     double rate = 0.0;         // low-pass mean firing rate (Hz)
@@ -252,6 +273,15 @@ nsx_status build_csr_device(nsx_ctx* ctx, uint32_t* d_src, uint32_t* d_dst, floa
 nsx_status build_csc_device(nsx_ctx* ctx);
 nsx_status validate_csr_device(nsx_ctx* ctx);
 hipError_t launch_update_surrogate(nsx_ctx* ctx, int32_t off);
+// Binned delivery: every event still in a bucket buffer added into its ring slot, the buffers
+// marked empty (the ring then holds all pending input, as with atomic delivery).
+nsx_status bin_materialize(nsx_ctx* ctx);
+// Mark every bucket buffer empty (after the ring was overwritten).
+nsx_status bin_clear(nsx_ctx* ctx);
+// (Re)allocate the bucket buffers for the current network, or release them.
+nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t tile_entries);
+// The loaded network fits binned delivery with tiles of tile_entries entries.
+bool bin_supported(const nsx_ctx* ctx, uint32_t tile_entries);
 }  // namespace nsx
 
 #define NSX_HIP(ctx, expr)                                          \

@@ -169,6 +169,11 @@ struct RowChunk {
   uint32_t incl[64];
   uint32_t src[64];
   uint64_t beg[64];
+  float pre_v[64];       // pair STDP: the row source's pre record (v, v_prev, s_prev), loaded
+  float pre_vp[64];      // once per row rather than once per slot
+  int32_t pre_sp[64];
+  uint32_t it_end[64];   // batched rows: inclusive prefix of the items' row counts
+  uint32_t it_base[64];  // and each item's first spike-segment index
 };
 
 // Inclusive wave scan of deg; stores prefix / start / source of each lane's row in LDS and
@@ -253,14 +258,110 @@ __device__ inline float pair_flush(const StepParams& p, const DevPtrs& s, float 
 // Propagation work item: part q of neuron block b.  PL: STDP applied on the spiking rows' E->E
 // slots -- pair: the target's pending depressions, potentiation by the post trace (then
 // depression if the target fired at t); triplet: depression by the target's last spike.
-template <int PL>
+// Binned delivery of one event: its slot in the step's bucket buffer comes from the
+// workgroup's LDS cursor of bucket (d - 1, target tile); the record is the target's LDS entry
+// in its tile's accumulator (channel-major) and the delivered weight.
+// A slot that does not exist (live = false) takes no place: its cursor add is 0 and nothing is
+// stored, but its loads stay unconditional so that both slots' loads are in flight together.
+__device__ inline void deliver_bin(const StepParams& p, uint32_t* pos, uint2* ev, uint32_t d,
+                                   uint32_t j, uint32_t c, float w, bool live) {
+  const uint32_t bin = (d - 1u) * p.bin_ntiles + (j >> p.bin_tshift);
+  const uint32_t entry = (c << p.bin_tshift) | (j & ((1u << p.bin_tshift) - 1u));
+  const uint32_t k = atomicAdd(&pos[bin], live ? 1u : 0u);
+  if (live) ev[k] = make_uint2(entry, __float_as_uint(w));
+}
+
+// The slots of one row chunk (chunk_setup done): propagation and the STDP at the pre spike.
+// Returns the potentiated / depressed-at-pre slot count of this lane.
+template <int PL, bool BIN>
+__device__ unsigned long long propagate_chunk(const StepParams& p, const DevPtrs& s,
+                                              const RowChunk& rc, int32_t t, uint32_t total,
+                                              uint32_t lane, uint32_t* pos, uint2* bev) {
+  const uint32_t tm = (uint32_t)t % p.D;
+  const size_t plane = (size_t)p.ch * p.n;
+  unsigned long long ltp = 0;
+  uint32_t r = 0;
+  for (uint32_t f = lane; f < total; f += 128) {
+    // two slots per lane: both sets of loads are issued before the atomics
+    const bool two = f + 64 < total;
+    const uint64_t k0 = chunk_slot(rc, f, r);
+    const uint32_t src0 = rc.src[r];
+    uint32_t r1 = r;
+    const uint64_t k1 = two ? chunk_slot(rc, f + 64, r1) : k0;
+    const uint32_t src1 = rc.src[r1];
+    const uint32_t j0 = s.col[k0], j1 = s.col[k1];
+    float w0 = s.w[k0], w1 = s.w[k1];
+    const uint32_t d0 = s.delay[k0], d1 = s.delay[k1];
+    const bool pot0 = PL != NSX_PLAST_NONE && src0 < p.n_exc && j0 < p.n_exc;
+    const bool pot1 = PL != NSX_PLAST_NONE && two && src1 < p.n_exc && j1 < p.n_exc;
+    PostRec pr0{0.0f, kNeverStep, kNeverStep, 0u, {}}, pr1{0.0f, kNeverStep, kNeverStep, 0u, {}};
+    if (pot0) pr0 = s.post[j0];  // 32 bytes: trace, last spike, pending spikes
+    if (pot1) pr1 = s.post[j1];
+    TraceRec a0{}, a1{};
+    if (PL == NSX_PLAST_PAIR) {  // the target's pending depressions precede the delivery
+      if (pot0) {
+        a0 = TraceRec{rc.pre_v[r], 0, rc.pre_vp[r], rc.pre_sp[r]};
+        w0 = pair_pending(p, s, w0, pr0, a0, t);
+      }
+      if (pot1) {
+        a1 = TraceRec{rc.pre_v[r1], 0, rc.pre_vp[r1], rc.pre_sp[r1]};
+        w1 = pair_pending(p, s, w1, pr1, a1, t);
+      }
+    }
+    uint32_t sl0 = tm + d0, sl1 = tm + d1;
+    sl0 = sl0 >= p.D ? sl0 - p.D : sl0;
+    sl1 = sl1 >= p.D ? sl1 - p.D : sl1;
+    const uint32_t ch0 = (p.ch > 1 && src0 >= p.n_exc) ? 1u : 0u;
+    const uint32_t ch1 = (p.ch > 1 && src1 >= p.n_exc) ? 1u : 0u;
+    if (BIN && pos) {
+      deliver_bin(p, pos, bev, d0, j0, ch0, w0, true);
+      deliver_bin(p, pos, bev, d1, j1, ch1, w1, two);
+    } else {
+      atomicAdd((unsigned long long*)(s.ring + sl0 * plane + (size_t)ch0 * p.n + j0),
+                (unsigned long long)quantize(w0, p.scale_in));
+      if (two)
+        atomicAdd((unsigned long long*)(s.ring + sl1 * plane + (size_t)ch1 * p.n + j1),
+                  (unsigned long long)quantize(w1, p.scale_in));
+    }
+    if (PL == NSX_PLAST_TRIPLET && (pot0 || pot1)) {
+      // pre spike at t: depression by the target's last spike (same step included)
+      if (pot0) {
+        const float x = trace_at(s.mtab, p.mtab_n, t - pr0.last);
+        const float dlt = p.a_minus * x;
+        const float wn = w0 - dlt;
+        s.w[k0] = clampf_(wn, p.w_min, p.w_max);
+        ltp += 1;
+      }
+      if (pot1) {
+        const float x = trace_at(s.mtab, p.mtab_n, t - pr1.last);
+        const float dlt = p.a_minus * x;
+        const float wn = w1 - dlt;
+        s.w[k1] = clampf_(wn, p.w_min, p.w_max);
+        ltp += 1;
+      }
+    }
+    if (PL == NSX_PLAST_PAIR && (pot0 || pot1)) {
+      if (pot0) {
+        s.w[k0] = pair_at_pre(p, s, w0, pr0, a0, t);
+        ltp += 1;
+      }
+      if (pot1) {
+        s.w[k1] = pair_at_pre(p, s, w1, pr1, a1, t);
+        ltp += 1;
+      }
+    }
+    r = r1;
+  }
+  return ltp;
+}
+
+template <int PL, bool BIN>
 __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
-                               uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane) {
+                               uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane,
+                               uint32_t* pos, uint2* bev) {
   const uint32_t P = p.prop_split;
   if (cnt <= q) return;
   const uint32_t mine = (cnt - q + P - 1) / P;  // rows q, q+P, ... of the segment
-  const uint32_t tm = (uint32_t)t % p.D;
-  const size_t plane = (size_t)p.ch * p.n;
   unsigned long long ev = 0, ltp = 0;
   for (uint32_t c0 = 0; c0 < mine; c0 += 64) {
     const uint32_t nr = min(64u, mine - c0);
@@ -271,75 +372,15 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
       beg = s.row_ptr[i];
       deg = (uint32_t)(s.row_ptr[i + 1] - beg);
     }
+    if (PL == NSX_PLAST_PAIR && lane < nr && i < p.n_exc) {
+      const TraceRec a = s.pre[i];
+      rc.pre_v[lane] = a.v;
+      rc.pre_vp[lane] = a.v_prev;
+      rc.pre_sp[lane] = a.s_prev;
+    }
     const uint32_t total = chunk_setup(rc, lane, deg, beg, i);
     ev += total;
-    uint32_t r = 0;
-    for (uint32_t f = lane; f < total; f += 128) {
-      // two slots per lane: both sets of loads are issued before the atomics
-      const bool two = f + 64 < total;
-      const uint64_t k0 = chunk_slot(rc, f, r);
-      const uint32_t src0 = rc.src[r];
-      uint32_t r1 = r;
-      const uint64_t k1 = two ? chunk_slot(rc, f + 64, r1) : k0;
-      const uint32_t src1 = rc.src[r1];
-      const uint32_t j0 = s.col[k0], j1 = s.col[k1];
-      float w0 = s.w[k0], w1 = s.w[k1];
-      const uint32_t d0 = s.delay[k0], d1 = s.delay[k1];
-      const bool pot0 = PL != NSX_PLAST_NONE && src0 < p.n_exc && j0 < p.n_exc;
-      const bool pot1 = PL != NSX_PLAST_NONE && two && src1 < p.n_exc && j1 < p.n_exc;
-      PostRec pr0{0.0f, kNeverStep, kNeverStep, 0u, {}}, pr1{0.0f, kNeverStep, kNeverStep, 0u, {}};
-      if (pot0) pr0 = s.post[j0];  // 32 bytes: trace, last spike, pending spikes
-      if (pot1) pr1 = s.post[j1];
-      TraceRec a0{}, a1{};
-      if (PL == NSX_PLAST_PAIR) {  // the target's pending depressions precede the delivery
-        if (pot0) {
-          a0 = s.pre[src0];
-          w0 = pair_pending(p, s, w0, pr0, a0, t);
-        }
-        if (pot1) {
-          a1 = s.pre[src1];
-          w1 = pair_pending(p, s, w1, pr1, a1, t);
-        }
-      }
-      uint32_t sl0 = tm + d0, sl1 = tm + d1;
-      sl0 = sl0 >= p.D ? sl0 - p.D : sl0;
-      sl1 = sl1 >= p.D ? sl1 - p.D : sl1;
-      const uint32_t ch0 = (p.ch > 1 && src0 >= p.n_exc) ? 1u : 0u;
-      const uint32_t ch1 = (p.ch > 1 && src1 >= p.n_exc) ? 1u : 0u;
-      atomicAdd((unsigned long long*)(s.ring + sl0 * plane + (size_t)ch0 * p.n + j0),
-                (unsigned long long)quantize(w0, p.scale_in));
-      if (two)
-        atomicAdd((unsigned long long*)(s.ring + sl1 * plane + (size_t)ch1 * p.n + j1),
-                  (unsigned long long)quantize(w1, p.scale_in));
-      if (PL == NSX_PLAST_TRIPLET && (pot0 || pot1)) {
-        // pre spike at t: depression by the target's last spike (same step included)
-        if (pot0) {
-          const float x = trace_at(s.mtab, p.mtab_n, t - pr0.last);
-          const float dlt = p.a_minus * x;
-          const float wn = w0 - dlt;
-          s.w[k0] = clampf_(wn, p.w_min, p.w_max);
-          ltp += 1;
-        }
-        if (pot1) {
-          const float x = trace_at(s.mtab, p.mtab_n, t - pr1.last);
-          const float dlt = p.a_minus * x;
-          const float wn = w1 - dlt;
-          s.w[k1] = clampf_(wn, p.w_min, p.w_max);
-          ltp += 1;
-        }
-      }
-      if (PL == NSX_PLAST_PAIR && (pot0 || pot1)) {
-        if (pot0) {
-          s.w[k0] = pair_at_pre(p, s, w0, pr0, a0, t);
-          ltp += 1;
-        }
-        if (pot1) {
-          s.w[k1] = pair_at_pre(p, s, w1, pr1, a1, t);
-          ltp += 1;
-        }
-      }
-      r = r1;
-    }
+    ltp += propagate_chunk<PL, BIN>(p, s, rc, t, total, lane, pos, bev);
     wave_sync();
   }
   if (PL != NSX_PLAST_NONE) {
@@ -473,18 +514,19 @@ __device__ inline void decode_item(uint32_t wk, uint32_t both, uint32_t nprop, u
   q = item - b * P;
 }
 
-template <int PL>
-__global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, int32_t off) {
-  __shared__ RowChunk sh[kBlock / 64];
-  const int32_t t = *s.tbase + off;
-  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+// The work-item loop of the synapse pass (and of binned delivery's counting pass, which must
+// visit exactly the same propagation items in the same workgroups): wavefront w of the grid
+// owns items w, w + W, ...; fn(prop, b, q, cnt) runs for the items whose block spiked.
+template <int PL, uint32_t NT, typename Fn>
+__device__ inline void for_items(const StepParams& p, const DevPtrs& s, Fn&& fn) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * NT + threadIdx.x) >> 6);
   const uint32_t P = p.prop_split;
   const uint32_t nprop = p.nblk * P;
   const uint32_t nltd = PL != NSX_PLAST_NONE ? ((p.n_exc + kBlock - 1) / kBlock) * P : 0u;
   const uint32_t both = 2 * min(nprop, nltd);
   const uint32_t nwork = nprop + nltd;
-  const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+  const uint32_t nwaves = (gridDim.x * NT) >> 6;
   for (uint64_t base = wave; base < nwork; base += 64ull * nwaves) {
     const uint64_t mine = base + (uint64_t)lane * nwaves;
     uint32_t c = 0, qq = 0;
@@ -502,11 +544,390 @@ __global__ __launch_bounds__(kBlock) void k_synapses(StepParams p, DevPtrs s, in
       uint32_t b, q;
       decode_item((uint32_t)(base + (uint64_t)l * nwaves), both, nprop, nltd, P, prop, b, q);
       const uint32_t cnt = __builtin_amdgcn_readfirstlane(__shfl(c, (int)l));
-      if (prop)
-        propagate_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
-      else if (PL != NSX_PLAST_NONE)
-        depress_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
+      fn(prop, b, q, cnt);
     }
+  }
+}
+
+// The same items and wavefronts as for_items, but a batch's propagation items are handed over
+// together: prop(mine, b, q, cnt) runs once per batch on every lane (lane l: item l of the batch,
+// mine = it is a propagation item with rows), then post(b, q, cnt) once per post-spike item.
+template <int PL, uint32_t NT, typename PropFn, typename PostFn>
+__device__ inline void for_batches(const StepParams& p, const DevPtrs& s, PropFn&& prop,
+                                   PostFn&& post) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * NT + threadIdx.x) >> 6);
+  const uint32_t P = p.prop_split;
+  const uint32_t nprop = p.nblk * P;
+  const uint32_t nltd = PL != NSX_PLAST_NONE ? ((p.n_exc + kBlock - 1) / kBlock) * P : 0u;
+  const uint32_t both = 2 * min(nprop, nltd);
+  const uint32_t nwork = nprop + nltd;
+  const uint32_t nwaves = (gridDim.x * NT) >> 6;
+  for (uint64_t base = wave; base < nwork; base += 64ull * nwaves) {
+    const uint64_t item = base + (uint64_t)lane * nwaves;
+    uint32_t c = 0, qq = 0, bb = 0;
+    bool pr = false;
+    if (item < nwork) {
+      decode_item((uint32_t)item, both, nprop, nltd, P, pr, bb, qq);
+      c = s.bcnt[bb];
+    }
+    const bool live = item < nwork && c > qq;
+    prop(live && pr, bb, qq, c);
+    unsigned long long act = __ballot(live && !pr);
+    while (act) {
+      const uint32_t l = (uint32_t)(__ffsll((long long)act) - 1);
+      act &= act - 1ull;
+      const uint32_t b = __builtin_amdgcn_readfirstlane(__shfl(bb, (int)l));
+      const uint32_t q = __builtin_amdgcn_readfirstlane(__shfl(qq, (int)l));
+      const uint32_t cnt = __builtin_amdgcn_readfirstlane(__shfl(c, (int)l));
+      post(b, q, cnt);
+    }
+  }
+}
+
+// Seven wavefronts per SIMD (72 VGPRs) for the pair-STDP instance: fits without spills.
+template <int PL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
+void k_synapses(StepParams p, DevPtrs s, int32_t off) {
+  __shared__ RowChunk sh[kBlock / 64];
+  const int32_t t = *s.tbase + off;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  for_items<PL, kBlock>(p, s, [&](bool prop, uint32_t b, uint32_t q, uint32_t cnt) {
+    if (prop)
+      propagate_item<PL, false>(p, s, sh[wid], t, b, q, cnt, lane, nullptr, nullptr);
+    else if (PL != NSX_PLAST_NONE)
+      depress_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
+  });
+}
+
+// ---- binned delivery ------------------------------------------------------------------------
+// Per step t, after the update: (1) k_bin_hist counts each workgroup's propagation events per
+// bucket (delay, target tile) in LDS and writes its row of counts; (2) k_bin_scan turns the
+// [groups][buckets] counts into per-group offsets within each bucket and bucket totals;
+// (3) k_synapses_bin runs the synapse pass, each event stored at its bucket's start + its
+// group's offset + an LDS cursor; (4) k_bin_arrive, one workgroup per target tile, sums the
+// tile's buckets of every emission step whose events arrive at t + 1 into LDS int64
+// accumulators and adds them into ring slot (t + 1) mod D.  Integer sums do not depend on the
+// order of the events, so the ring equals the one atomic delivery builds.
+
+// Counting pass over the slots of one row chunk: four slots per lane per round; every load is
+// unconditional (a slot past the end repeats the first and counts 0) so that all eight are in
+// flight together.
+__device__ inline void hist_chunk(const StepParams& p, const DevPtrs& s, const RowChunk& rc,
+                                  uint32_t* h, uint32_t total, uint32_t lane) {
+  uint32_t r = 0;
+  for (uint32_t f = lane; f < total; f += 256) {
+    uint64_t k[4];
+    uint32_t add[4];
+    k[0] = chunk_slot(rc, f, r);
+    add[0] = 1u;
+#pragma unroll
+    for (uint32_t u = 1; u < 4; ++u) {
+      add[u] = f + 64u * u < total ? 1u : 0u;
+      k[u] = add[u] ? chunk_slot(rc, f + 64u * u, r) : k[0];
+    }
+    uint32_t j[4], d[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      j[u] = s.col[k[u]];
+      d[u] = s.delay[k[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u)
+      atomicAdd(&h[(d[u] - 1u) * p.bin_ntiles + (j[u] >> p.bin_tshift)], add[u]);
+  }
+}
+
+// Spiking rows of all the propagation items of a wavefront's batch (lane l: item (b, q, cnt),
+// `mine` when it is a propagation item whose segment part has rows), walked together in chunks
+// of 64 rows: far fewer dependent memory rounds than one item at a time when items are small.
+// fn(rc, total) runs once per chunk after chunk_setup; returns the rows' slot count.
+template <typename Fn>
+__device__ inline unsigned long long batch_rows(const StepParams& p, const DevPtrs& s,
+                                                RowChunk& rc, uint32_t lane, bool mine,
+                                                uint32_t b, uint32_t q, uint32_t cnt,
+                                                bool pre_rec, Fn&& fn) {
+  const uint32_t P = p.prop_split;
+  const uint32_t m = mine ? (cnt - q + P - 1) / P : 0u;
+  uint32_t incl = m;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  const uint32_t R = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+  if (R == 0) return 0;
+  rc.it_end[lane] = incl;
+  rc.it_base[lane] = b * kBlock + q;
+  wave_sync();
+  unsigned long long ev = 0;
+  for (uint32_t r0 = 0; r0 < R; r0 += 64) {
+    const uint32_t r = r0 + lane;
+    uint32_t i = 0, deg = 0;
+    uint64_t beg = 0;
+    if (r < R) {
+      uint32_t l = 0;  // item of row r: lanes whose inclusive prefix is <= r come before it
+#pragma unroll
+      for (uint32_t st = 32; st > 0; st >>= 1)
+        if (rc.it_end[l + st - 1] <= r) l += st;
+      const uint32_t k = r - (l ? rc.it_end[l - 1] : 0u);
+      i = s.spikes[rc.it_base[l] + k * P];
+      beg = s.row_ptr[i];
+      deg = (uint32_t)(s.row_ptr[i + 1] - beg);
+      if (pre_rec && i < p.n_exc) {
+        const TraceRec a = s.pre[i];
+        rc.pre_v[lane] = a.v;
+        rc.pre_vp[lane] = a.v_prev;
+        rc.pre_sp[lane] = a.s_prev;
+      }
+    }
+    const uint32_t total = chunk_setup(rc, lane, deg, beg, i);
+    ev += total;
+    fn(total);
+    wave_sync();
+  }
+  return ev;
+}
+
+template <int PL>
+__global__ __launch_bounds__(kBinThreads) void k_bin_hist(StepParams p, DevPtrs s) {
+  __shared__ RowChunk sh[kBinThreads / 64];
+  __shared__ uint32_t h[kMaxBins];
+  const uint32_t nb = p.bin_nbins, lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  for (uint32_t k = threadIdx.x; k < nb; k += kBinThreads) h[k] = 0u;
+  __syncthreads();
+  RowChunk& rc = sh[wid];
+  for_batches<PL, kBinThreads>(
+      p, s,
+      [&](bool mine, uint32_t b, uint32_t q, uint32_t cnt) {
+        batch_rows(p, s, rc, lane, mine, b, q, cnt, false,
+                   [&](uint32_t total) { hist_chunk(p, s, rc, h, total, lane); });
+      },
+      [&](uint32_t, uint32_t, uint32_t) {});
+  __syncthreads();
+  uint32_t* row = s.bin_hist + (size_t)blockIdx.x * nb;
+  for (uint32_t k = threadIdx.x; k < nb; k += kBinThreads) row[k] = h[k];
+}
+
+// 16 buckets per workgroup of 256 threads: thread (column c, row group rg) holds its column's
+// counts of groups [rg * per, (rg + 1) * per) in registers (per <= 64, groups <= 1024), the 16
+// row-group sums of a column are scanned in LDS, and the exclusive offsets are written back in
+// place; the column totals go to bin_tot.  Each load instruction reads 64-byte row pieces.
+constexpr uint32_t kScanThreads = 256;
+__global__ __launch_bounds__(kScanThreads) void k_bin_scan(StepParams p, DevPtrs s) {
+  __shared__ uint32_t part[16][16];
+  const uint32_t nb = p.bin_nbins, G = p.bin_groups;
+  const uint32_t c = threadIdx.x & 15u, rg = threadIdx.x >> 4;
+  const uint32_t col = blockIdx.x * 16u + c;
+  const uint32_t per = (G + 15u) / 16u;
+  const uint32_t r0 = rg * per;
+  uint32_t v[64], sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 64; ++k) {
+    const uint32_t r = r0 + k;
+    v[k] = (k < per && r < G && col < nb) ? s.bin_hist[(size_t)r * nb + col] : 0u;
+    sum += v[k];
+  }
+  part[rg][c] = sum;
+  __syncthreads();
+  uint32_t run = 0, tot = 0;
+#pragma unroll
+  for (uint32_t g = 0; g < 16; ++g) {
+    const uint32_t x = part[g][c];
+    run += g < rg ? x : 0u;
+    tot += x;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < 64; ++k) {
+    const uint32_t r = r0 + k;
+    if (k < per && r < G && col < nb) s.bin_hist[(size_t)r * nb + col] = run;
+    run += v[k];
+  }
+  if (rg == 0 && col < nb) s.bin_tot[col] = tot;
+}
+
+// Bucket starts of step t (exclusive scan of the totals, 8 buckets per thread) plus this
+// group's offsets, as LDS cursors.  Group 0 publishes the starts as the step's table; a step
+// whose events exceed the buffer publishes an empty table and returns false (atomic delivery).
+__device__ bool bin_positions(const StepParams& p, const DevPtrs& s, uint32_t* pos,
+                              uint32_t* wsum, int32_t t) {
+  const uint32_t nb = p.bin_nbins, tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+  uint32_t v[8], sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t b = tid * 8 + k;
+    v[k] = b < nb ? s.bin_tot[b] : 0u;
+    sum += v[k];
+  }
+  uint32_t incl = sum;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  uint32_t pre = 0, total = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kBinThreads / 64; ++q) {
+    const uint32_t x = wsum[q];
+    pre += q < wid ? x : 0u;
+    total += x;
+  }
+  uint32_t run = pre + incl - sum;
+  const bool use = total <= p.bin_cap;
+  uint32_t* tab = s.bin_tab + (size_t)((uint32_t)t % p.D) * (nb + 1);
+  const uint32_t* off = s.bin_hist + (size_t)blockIdx.x * nb;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t b = tid * 8 + k;
+    if (b < nb) {
+      pos[b] = run + off[b];
+      if (blockIdx.x == 0) tab[b] = use ? run : 0u;
+    }
+    run += v[k];
+  }
+  if (blockIdx.x == 0 && tid == 0) tab[nb] = use ? total : 0u;
+  __syncthreads();
+  return use;
+}
+
+template <int PL>
+__global__ __launch_bounds__(kBinThreads)
+void k_synapses_bin(StepParams p, DevPtrs s, int32_t off) {
+  __shared__ RowChunk sh[kBinThreads / 64];
+  __shared__ uint32_t pos[kMaxBins];
+  __shared__ uint32_t wsum[kBinThreads / 64];
+  const int32_t t = *s.tbase + off;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  const bool use = bin_positions(p, s, pos, wsum, t);
+  uint32_t* cur = use ? pos : nullptr;
+  uint2* bev = s.bin_ev + (size_t)((uint32_t)t % p.D) * p.bin_cap;
+  RowChunk& rc = sh[wid];
+  for_batches<PL, kBinThreads>(
+      p, s,
+      [&](bool mine, uint32_t b, uint32_t q, uint32_t cnt) {
+        unsigned long long ltp = 0;
+        const unsigned long long ev = batch_rows(
+            p, s, rc, lane, mine, b, q, cnt, PL == NSX_PLAST_PAIR, [&](uint32_t total) {
+              ltp += propagate_chunk<PL, true>(p, s, rc, t, total, lane, cur, bev);
+            });
+        if (PL != NSX_PLAST_NONE) {
+#pragma unroll
+          for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
+        }
+        // totals only: a batch's counts go to the block of its first propagation item
+        const unsigned long long any = __ballot(mine);
+        if (any) {
+          const uint32_t b0 = __builtin_amdgcn_readfirstlane(
+              __shfl(b, (int)(__ffsll((long long)any) - 1)));
+          if (lane == 0) {
+            atomicAdd(&s.part[kPartEvents * p.nblk + b0], ev);
+            if (PL != NSX_PLAST_NONE) atomicAdd(&s.part[kPartLtp * p.nblk + b0], ltp);
+          }
+        }
+      },
+      [&](uint32_t b, uint32_t q, uint32_t cnt) {
+        if (PL != NSX_PLAST_NONE) depress_item<PL>(p, s, rc, t, b, q, cnt, lane);
+      });
+}
+
+// Arrival at step ta = *tbase + off of the events emitted at steps e in [ta - D + 1, ta - 1]
+// with e <= emax (emission steps after emax have not run), for target tile blockIdx.x and
+// tile size TE (entries = channels x neurons).  The tile's bucket ranges are staged in LDS and
+// walked as one flattened list, eight events per thread in flight.
+template <uint32_t TE>
+__global__ __launch_bounds__(kBinThreads) void k_bin_arrive(StepParams p, DevPtrs s, int32_t off,
+                                                            int32_t emax) {
+  __shared__ long long acc[TE];
+  __shared__ uint32_t seg_lo[kMaxArriveSegs];
+  __shared__ uint32_t seg_end[kMaxArriveSegs];  // inclusive prefix of segment lengths
+  __shared__ uint32_t seg_buf[kMaxArriveSegs];
+  const int32_t ta = *s.tbase + off;
+  const uint32_t nb = p.bin_nbins, T = blockIdx.x, nseg = p.D - 1u;
+  for (uint32_t k = threadIdx.x; k < TE; k += kBinThreads) acc[k] = 0;
+  // segment d - 1: bucket (d - 1, T) of emission step ta - d
+  uint32_t len = 0, lo = 0, bufi = 0;
+  if (threadIdx.x < nseg) {
+    const int32_t e = ta - (int32_t)(threadIdx.x + 1u);
+    if (e >= 0 && e <= emax) {
+      bufi = (uint32_t)e % p.D;
+      const uint32_t* tab = s.bin_tab + (size_t)bufi * (nb + 1);
+      const uint32_t bin = threadIdx.x * p.bin_ntiles + T;
+      lo = tab[bin];
+      len = tab[bin + 1] - lo;
+    }
+  }
+  // inclusive scan of len over the (at most kMaxArriveSegs) segments
+  __shared__ uint32_t wtot[kBinThreads / 64];
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  uint32_t incl = len;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wtot[wid] = incl;
+  __syncthreads();
+  uint32_t pre = 0, total = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kBinThreads / 64; ++q) {
+    const uint32_t x = wtot[q];
+    pre += q < wid ? x : 0u;
+    total += x;
+  }
+  if (threadIdx.x < nseg) {
+    seg_lo[threadIdx.x] = lo;
+    seg_end[threadIdx.x] = pre + incl;
+    seg_buf[threadIdx.x] = bufi;
+  }
+  __syncthreads();
+  const uint2* ev = s.bin_ev;
+  const size_t cap = p.bin_cap;
+  // Loads are unconditional (an index past the end is clamped to the last event) so that the
+  // eight of a round are in flight together; the segment of a flat index is found by walking
+  // forward from the previous one (indices grow by 1024, segments are about that long).
+  for (uint32_t f0 = 0; f0 < total; f0 += 8 * kBinThreads) {
+    uint2 x[8];
+    uint32_t a = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+      const uint32_t f = min(f0 + u * kBinThreads + threadIdx.x, total - 1u);
+      while (seg_end[a] <= f) ++a;  // stops at the segment holding f (f < seg_end[nseg-1])
+      const uint32_t start = a ? seg_end[a - 1] : 0u;
+      x[u] = ev[(size_t)seg_buf[a] * cap + seg_lo[a] + (f - start)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 8; ++u) {
+      const uint32_t f = f0 + u * kBinThreads + threadIdx.x;
+      if (f < total)
+        atomicAdd((unsigned long long*)&acc[x[u].x],
+                  (unsigned long long)quantize(__uint_as_float(x[u].y), p.scale_in));
+    }
+  }
+  __syncthreads();
+  // ring slot ta += tile accumulators, four entries per thread per round, loads first
+  const uint32_t tn = 1u << p.bin_tshift;
+  const uint32_t slot = (uint32_t)ta % p.D;
+  long long* ring = s.ring + (size_t)slot * p.ch * p.n;
+  for (uint32_t k0 = 0; k0 < TE; k0 += 4 * kBinThreads) {
+    size_t idx[4];
+    long long r[4], add[4];
+    bool wr[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t k = k0 + u * kBinThreads + threadIdx.x;
+      const uint32_t c = k >> p.bin_tshift;
+      const uint64_t j = (uint64_t)T * tn + (k & (tn - 1u));
+      const bool in = c < p.ch && j < p.n;
+      add[u] = acc[k];
+      wr[u] = in && add[u] != 0;
+      idx[u] = in ? (size_t)c * p.n + j : 0u;
+      r[u] = ring[idx[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u)
+      if (wr[u]) ring[idx[u]] = r[u] + add[u];
   }
 }
 
@@ -550,6 +971,20 @@ uint32_t syn_grid(const nsx_ctx* ctx) {
 }
 
 // Enqueue the launches of one step with offset `off` from *tbase.
+// Arrival of the bucketed events at step *tbase + off (emission steps up to emax).
+hipError_t launch_arrive(nsx_ctx* ctx, int32_t off, int32_t emax) {
+  const StepParams& p = ctx->p;
+  const dim3 g(p.bin_ntiles), b(kBinThreads);
+  switch (ctx->bin_te) {
+    case 4096: hipLaunchKernelGGL(k_bin_arrive<4096>, g, b, 0, ctx->stream, p, ctx->d, off, emax); break;
+    case 8192: hipLaunchKernelGGL(k_bin_arrive<8192>, g, b, 0, ctx->stream, p, ctx->d, off, emax); break;
+    default: hipLaunchKernelGGL(k_bin_arrive<16384>, g, b, 0, ctx->stream, p, ctx->d, off, emax); break;
+  }
+  return hipGetLastError();
+}
+
+// Enqueue the launches of one step with offset `off` from *tbase.  Profiling events:
+// [0] update [1] binned counting passes [2] synapse pass [3] binned arrival [4].
 hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
   const StepParams& p = ctx->p;
   const int pl = (int)ctx->cfg.plasticity;
@@ -564,16 +999,41 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
     hipLaunchKernelGGL(upd, dim3(gu), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
   }
   if (evs) (void)hipEventRecord(evs[1], ctx->stream);
-  if (pl == NSX_PLAST_PAIR)
-    hipLaunchKernelGGL(k_synapses<NSX_PLAST_PAIR>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
-                       ctx->d, off);
-  else if (pl == NSX_PLAST_TRIPLET)
-    hipLaunchKernelGGL(k_synapses<NSX_PLAST_TRIPLET>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
-                       ctx->d, off);
-  else
-    hipLaunchKernelGGL(k_synapses<NSX_PLAST_NONE>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
-                       ctx->d, off);
-  if (evs) (void)hipEventRecord(evs[2], ctx->stream);
+  if (p.bin_on) {
+    const dim3 gb(p.bin_groups), bb(kBinThreads);
+    if (pl == NSX_PLAST_PAIR)
+      hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_PAIR>, gb, bb, 0, ctx->stream, p, ctx->d);
+    else if (pl == NSX_PLAST_TRIPLET)
+      hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_TRIPLET>, gb, bb, 0, ctx->stream, p, ctx->d);
+    else
+      hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_NONE>, gb, bb, 0, ctx->stream, p, ctx->d);
+    hipLaunchKernelGGL(k_bin_scan, dim3((p.bin_nbins + 15) / 16), dim3(kScanThreads), 0,
+                       ctx->stream, p, ctx->d);
+    if (evs) (void)hipEventRecord(evs[2], ctx->stream);
+    if (pl == NSX_PLAST_PAIR)
+      hipLaunchKernelGGL(k_synapses_bin<NSX_PLAST_PAIR>, gb, bb, 0, ctx->stream, p, ctx->d, off);
+    else if (pl == NSX_PLAST_TRIPLET)
+      hipLaunchKernelGGL(k_synapses_bin<NSX_PLAST_TRIPLET>, gb, bb, 0, ctx->stream, p, ctx->d,
+                         off);
+    else
+      hipLaunchKernelGGL(k_synapses_bin<NSX_PLAST_NONE>, gb, bb, 0, ctx->stream, p, ctx->d, off);
+    if (evs) (void)hipEventRecord(evs[3], ctx->stream);
+    const hipError_t e = launch_arrive(ctx, off + 1, INT32_MAX);
+    if (e != hipSuccess) return e;
+  } else {
+    if (evs) (void)hipEventRecord(evs[2], ctx->stream);
+    if (pl == NSX_PLAST_PAIR)
+      hipLaunchKernelGGL(k_synapses<NSX_PLAST_PAIR>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
+                         ctx->d, off);
+    else if (pl == NSX_PLAST_TRIPLET)
+      hipLaunchKernelGGL(k_synapses<NSX_PLAST_TRIPLET>, dim3(gs), dim3(kBlock), 0, ctx->stream,
+                         p, ctx->d, off);
+    else
+      hipLaunchKernelGGL(k_synapses<NSX_PLAST_NONE>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
+                         ctx->d, off);
+    if (evs) (void)hipEventRecord(evs[3], ctx->stream);
+  }
+  if (evs) (void)hipEventRecord(evs[4], ctx->stream);
   return hipGetLastError();
 }
 
@@ -618,4 +1078,143 @@ nsx_status flush_pending(nsx_ctx* ctx, bool apply) {
   return NSX_OK;
 }
 
+// ---- binned delivery: host side ------------------------------------------------------------
+
+nsx_status bin_clear(nsx_ctx* ctx) {
+  if (!ctx->p.bin_on) return NSX_OK;
+  NSX_HIP(ctx, hipMemsetAsync(ctx->d.bin_tab, 0,
+                              (size_t)ctx->D * (ctx->p.bin_nbins + 1) * 4, ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return NSX_OK;
+}
+
+// The arrival of step t (the next step) has run; the events of steps up to t - 1 that arrive
+// at t + 1 .. t + D - 2 are still in the buffers.
+nsx_status bin_materialize(nsx_ctx* ctx) {
+  if (!ctx->p.bin_on) return NSX_OK;
+  const int64_t t = (int64_t)ctx->t;
+  for (int64_t ta = t + 1; t > 0 && ta <= t + ctx->D - 2; ++ta) {
+    const hipError_t e = launch_arrive(ctx, (int32_t)(ta - ctx->tbase_host), (int32_t)(t - 1));
+    if (e != hipSuccess) return hip_error(ctx, e, "binned delivery arrival");
+  }
+  return bin_clear(ctx);
+}
+
+// Tile shift, tile count and bucket count of binned delivery with te-entry tiles; false when
+// the network does not fit (buckets, ring slots, 32-bit event counts).
+static bool bin_shape(const nsx_ctx* ctx, uint32_t te, uint32_t& tshift, uint64_t& ntiles,
+                      uint64_t& nbins) {
+  tshift = (uint32_t)__builtin_ctz(te) - (ctx->ch == 2 ? 1u : 0u);
+  ntiles = (ctx->n + (1ull << tshift) - 1) >> tshift;
+  nbins = (uint64_t)(ctx->D - 1) * ntiles;
+  return nbins <= kMaxBins && (uint32_t)ctx->D - 1 <= kMaxArriveSegs && ctx->m < (1ull << 32);
+}
+
+bool bin_supported(const nsx_ctx* ctx, uint32_t te) {
+  uint32_t tshift;
+  uint64_t ntiles, nbins;
+  return ctx->has_network && bin_shape(ctx, te, tshift, ntiles, nbins);
+}
+
+static void bin_release(nsx_ctx* ctx) {
+  dev_free(ctx, ctx->d.bin_ev);
+  dev_free(ctx, ctx->d.bin_tab);
+  dev_free(ctx, ctx->d.bin_hist);
+  dev_free(ctx, ctx->d.bin_tot);
+  ctx->d.bin_ev = nullptr;
+  ctx->d.bin_tab = ctx->d.bin_hist = ctx->d.bin_tot = nullptr;
+  ctx->p.bin_on = 0;
+  ctx->bin_te = 0;
+}
+
+nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t te) {
+  if (mode != NSX_DELIVERY_ATOMIC && mode != NSX_DELIVERY_BINNED)
+    return set_error(ctx, NSX_E_INVALID, "delivery mode must be ATOMIC or BINNED");
+  if (te != 0 && te != 4096 && te != 8192 && te != 16384)
+    return set_error(ctx, NSX_E_INVALID, "tile_entries must be 0, 4096, 8192 or 16384");
+  nsx_status s = bin_materialize(ctx);  // pending events move into the ring
+  if (s) return s;
+  bin_release(ctx);
+  destroy_graph(ctx);
+  ctx->delivery = mode;
+  ctx->bin_cap_req = cap;
+  ctx->bin_te_req = te;
+  ctx->bin_dirty = false;
+  if (mode == NSX_DELIVERY_ATOMIC) return NSX_OK;
+  if (!ctx->has_network) {  // sized when a network is loaded
+    ctx->bin_dirty = true;
+    return NSX_OK;
+  }
+  const uint32_t D = (uint32_t)ctx->D;
+  const uint64_t m = ctx->m;
+  if (m >= (1ull << 32)) return set_error(ctx, NSX_E_INVALID, "binned delivery needs M < 2^32");
+  // automatic tile: the largest whose tiles still number at least 128 (arrival workgroups)
+  if (te == 0) {
+    te = 4096;
+    for (uint32_t c : {16384u, 8192u})
+      if ((ctx->n * ctx->ch + c - 1) / c >= 128) {
+        te = c;
+        break;
+      }
+  }
+  uint32_t tshift = 0;
+  uint64_t ntiles = 0, nbins = 0;
+  if (!bin_shape(ctx, te, tshift, ntiles, nbins))
+    return set_error(ctx, NSX_E_INVALID, "binned delivery needs (D - 1) * tiles <= 8192");
+  int cus = 0, per_cu = 0;
+  NSX_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const void* kfn = ctx->cfg.plasticity == NSX_PLAST_PAIR
+                        ? (const void*)k_synapses_bin<NSX_PLAST_PAIR>
+                        : ctx->cfg.plasticity == NSX_PLAST_TRIPLET
+                              ? (const void*)k_synapses_bin<NSX_PLAST_TRIPLET>
+                              : (const void*)k_synapses_bin<NSX_PLAST_NONE>;
+  NSX_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBinThreads, 0));
+  const uint32_t groups = (uint32_t)std::min(std::max(cus * std::max(per_cu, 1), 1), 1024);
+  if (cap == 0) {
+    cap = std::min<uint64_t>({m, 1ull << 24, (1ull << 31) / (8ull * D)});
+    cap = std::max<uint64_t>(cap, std::min<uint64_t>(m, 1ull << 16));
+  }
+  cap = std::max<uint64_t>(std::min<uint64_t>(cap, 0xFFFFFFFFull), 1);
+  DevPtrs& d = ctx->d;
+  if ((s = dev_alloc(ctx, (void**)&d.bin_ev, (size_t)D * cap * 8, false)) ||
+      (s = dev_alloc(ctx, (void**)&d.bin_tab, (size_t)D * (nbins + 1) * 4, true)) ||
+      (s = dev_alloc(ctx, (void**)&d.bin_hist, (size_t)groups * nbins * 4, true)) ||
+      (s = dev_alloc(ctx, (void**)&d.bin_tot, (size_t)nbins * 4, true))) {
+    bin_release(ctx);
+    return s;
+  }
+  StepParams& p = ctx->p;
+  p.bin_tshift = tshift;
+  p.bin_ntiles = (uint32_t)ntiles;
+  p.bin_nbins = (uint32_t)nbins;
+  p.bin_groups = groups;
+  p.bin_cap = (uint32_t)cap;
+  p.bin_on = 1;
+  ctx->bin_te = te;
+  return NSX_OK;
+}
+
 }  // namespace nsx
+
+using namespace nsx;
+
+extern "C" {
+
+nsx_status nsx_set_delivery(nsx_ctx* ctx, uint32_t mode, uint64_t cap_events,
+                            uint32_t tile_entries) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  nsx_status s = bin_configure(ctx, mode, cap_events, tile_entries);
+  if (s) {
+    ctx->delivery = NSX_DELIVERY_ATOMIC;
+    ctx->bin_dirty = false;
+  }
+  return s;
+}
+
+nsx_status nsx_get_delivery(nsx_ctx* ctx, uint32_t* mode) {
+  if (!ctx || !mode) return set_error(ctx, NSX_E_INVALID, "NULL argument");
+  *mode = ctx->delivery;
+  return NSX_OK;
+}
+
+}  // extern "C"

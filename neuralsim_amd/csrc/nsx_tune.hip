@@ -63,6 +63,8 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   if (trial_steps < 1) return set_error(ctx, NSX_E_INVALID, "trial_steps must be >= 1");
   if (ctx->cfg.model == NSX_MODEL_HH_SURROGATE && !(ctx->sur.has_params && ctx->sur.has_norm))
     return set_error(ctx, NSX_E_STATE, "surrogate parameters not set");
+  nsx_status s0 = bin_materialize(ctx);  // the snapshot's ring then holds all pending input
+  if (s0) return s0;
   const std::vector<Buf> bufs = mutable_buffers(ctx);
   size_t total = 0;
   for (const Buf& b : bufs) total += (b.bytes + 255) & ~(size_t)255;
@@ -74,6 +76,8 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   const int32_t tb0 = ctx->tbase_host;
   const bool prof = ctx->profiling;
   const uint32_t split0 = ctx->p.prop_split, graph0 = ctx->graph_steps;
+  const uint32_t deliv0 = ctx->delivery, te0 = ctx->bin_te_req;
+  const uint64_t cap0 = ctx->bin_cap_req;
   const uint32_t stats0 = ctx->p.stats_on;  // trials neither count nor fold statistics
   ctx->profiling = false;
   ctx->p.stats_on = 0;
@@ -81,8 +85,11 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   nsx_tune_report r{};
   double best = 1e300;
   uint32_t best_split = split0, best_graph = graph0 ? graph0 : 50;
-  auto trial = [&](uint32_t split, uint32_t graph) -> nsx_status {
-    nsx_status st = nsx_set_prop_split(ctx, split);
+  uint32_t best_te = deliv0 == NSX_DELIVERY_BINNED ? ctx->bin_te : 0u;  // 0: atomic
+  // te = 0: atomic delivery, else binned with that tile size
+  auto trial = [&](uint32_t split, uint32_t graph, uint32_t te) -> nsx_status {
+    nsx_status st = bin_configure(ctx, te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0, te);
+    if (!st) st = nsx_set_prop_split(ctx, split);
     if (!st) st = nsx_set_graph_steps(ctx, graph);
     if (st) return st;
     const uint64_t steps = graph ? (trial_steps + graph - 1) / graph * graph : trial_steps;
@@ -93,6 +100,7 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     if (r.n_trials < 32) {
       r.trial_split[r.n_trials] = split;
       r.trial_graph[r.n_trials] = graph;
+      r.trial_tile[r.n_trials] = te;
       r.us_per_step[r.n_trials] = us;
       ++r.n_trials;
     }
@@ -100,35 +108,50 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
       best = us;
       best_split = split;
       best_graph = graph;
+      best_te = te;
     }
     hipError_t e = copy_all(ctx, bufs, snap, false);
     if (e != hipSuccess) return hip_error(ctx, e, "autotune restore");
     ctx->t = t0;
     ctx->tbase_host = tb0;
-    return NSX_OK;
+    return bin_clear(ctx);  // the trial's events in flight are not part of the snapshot
   };
   // coordinate search: the split at a 50-step window, then the window at the best split
   static const uint32_t kSplits[] = {1, 2, 4, 8, 16};
   static const uint32_t kGraphs[] = {10, 25, 100, 200};
+  const uint32_t te_start = best_te;
   for (uint32_t sp : kSplits)
-    if ((s = trial(sp, 50))) break;
+    if ((s = trial(sp, 50, te_start))) break;
   const uint32_t sp_best = best_split;
   if (!s)
     for (uint32_t g : kGraphs)
-      if ((s = trial(sp_best, g))) break;
+      if ((s = trial(sp_best, g, te_start))) break;
+  // the other delivery: atomic, and binned where (D - 1) * tiles fits
+  const uint32_t sp2 = best_split, g2 = best_graph;
+  for (uint32_t te : {0u, 8192u, 16384u}) {
+    if (s || te == te_start) continue;
+    if (te && !bin_supported(ctx, te)) continue;
+    s = trial(sp2, g2, te);
+  }
   dev_free(ctx, snap);
   ctx->profiling = prof;
   ctx->p.stats_on = stats0;
   destroy_graph(ctx);
   if (s) {
+    (void)bin_configure(ctx, deliv0, cap0, te0);
     (void)nsx_set_prop_split(ctx, split0);
     (void)nsx_set_graph_steps(ctx, graph0);
     return s;
   }
+  if ((s = bin_configure(ctx, best_te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0,
+                         best_te)))
+    return s;
   if ((s = nsx_set_prop_split(ctx, best_split))) return s;
   if ((s = nsx_set_graph_steps(ctx, best_graph))) return s;
   r.prop_split = best_split;
   r.graph_steps = best_graph;
+  r.delivery = best_te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC;
+  r.tile_entries = best_te;
   r.best_us_per_step = best;
   if (rep) *rep = r;
   return NSX_OK;

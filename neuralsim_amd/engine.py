@@ -171,12 +171,29 @@ class SimulationEngine:
         self._chk(self._lib.nsx_set_prop_split(self._h, int(waves_per_block)),
                   "nsx_set_prop_split")
 
+    def set_delivery(self, mode, cap_events=0, tile_entries=0):
+        """How events reach the ring (abi.DELIVERY_ATOMIC / DELIVERY_BINNED); results are
+        bit-identical.  cap_events bounds one step's bucket buffer (more events: atomics that
+        step); tile_entries is 4096, 8192 or 16384 (0: automatic)."""
+        self._chk(self._lib.nsx_set_delivery(self._h, int(mode), int(cap_events),
+                                             int(tile_entries)), "nsx_set_delivery")
+
+    @property
+    def delivery(self):
+        m = C.c_uint32()
+        self._chk(self._lib.nsx_get_delivery(self._h, C.byref(m)), "nsx_get_delivery")
+        return m.value
+
     def autotune(self, trial_steps=100):
-        """Measure and set the fastest propagation split and graph window (state unchanged);
-        returns (chosen split, chosen window, [(split, window, us per step), ...])."""
+        """Measure and set the fastest propagation split, graph window and delivery (state
+        unchanged); returns (chosen split, chosen window, [(split, window, us per step, tile),
+        ...]) with tile 0 for atomic delivery, else the binned tile size; the chosen delivery is
+        then `delivery` / `last_tune.tile_entries`."""
         r = abi.NsxTuneReport()
         self._chk(self._lib.nsx_autotune(self._h, int(trial_steps), C.byref(r)), "nsx_autotune")
-        trials = [(r.trial_split[k], r.trial_graph[k], r.us_per_step[k]) for k in range(r.n_trials)]
+        self.last_tune = r
+        trials = [(r.trial_split[k], r.trial_graph[k], r.us_per_step[k], r.trial_tile[k])
+                  for k in range(r.n_trials)]
         return r.prop_split, r.graph_steps, trials
 
     def set_profiling(self, on):

@@ -15,8 +15,11 @@ def test_autotune_leaves_simulation_unchanged(nsx):
         a.step(120)
         split, graph, trials = a.autotune(40)
         print("chosen", split, graph, trials)
-        assert len(trials) == 9 and split in (1, 2, 4, 8, 16) and graph in (10, 25, 50, 100, 200)
+        # 5 splits, 4 more windows, then atomic vs binned with two tile sizes
+        assert len(trials) == 11 and split in (1, 2, 4, 8, 16) and graph in (10, 25, 50, 100, 200)
         assert all(t[2] > 0 for t in trials)
+        assert sorted(t[3] for t in trials[-3:]) == [0, 8192, 16384]
+        assert a.delivery == (1 if a.last_tune.tile_entries else 0)
         assert a.get_step() == 120
         a.step(130)
         ra, sa, wa = a.raster(), a.read_state(fields), a.weights()

@@ -1,12 +1,13 @@
 // nsx_tune.hip -- launch-shape autotuner.
 //
-// Chooses, by measurement on the context's own network and state, the two launch parameters the
+// Chooses, by measurement on the context's own network and state, the launch parameters the
 // step's cost depends on beyond the network itself: the propagation split (wavefronts that
 // share one neuron block's spike segment -- the row-split of busy blocks with high-degree
-// rows) and the graph window (steps per captured hipGraph launch).  The neuron block size is
-// fixed at 256 by the spike-segment layout.  Every trial starts from a device-side snapshot of
-// the mutable state and the state is restored afterwards, so tuning does not change the
-// simulation.
+// rows), the graph window (steps per captured hipGraph launch) and the delivery (ring atomics
+// or binned, with its tile size).  The neuron block size is
+// fixed at 256 by the spike-segment layout.  Every trial times the same steps from a
+// device-side snapshot of the mutable state (after its warm-up / capture the state is restored)
+// and the state is restored afterwards, so tuning does not change the simulation.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -95,6 +96,13 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     const uint64_t steps = graph ? (trial_steps + graph - 1) / graph * graph : trial_steps;
     nsx_step_stats ss{};
     if ((st = nsx_step(ctx, graph ? graph : 8, &ss))) return st;  // capture / warm up
+    // every trial times the same steps: back to the snapshot (the captured graph stays valid:
+    // the buffers keep their addresses and the step base is part of the snapshot)
+    hipError_t e0 = copy_all(ctx, bufs, snap, false);
+    if (e0 != hipSuccess) return hip_error(ctx, e0, "autotune restore");
+    ctx->t = t0;
+    ctx->tbase_host = tb0;
+    if ((st = bin_clear(ctx))) return st;
     if ((st = nsx_step(ctx, steps, &ss))) return st;
     const double us = ss.wall_ms * 1000.0 / (double)steps;
     if (r.n_trials < 32) {

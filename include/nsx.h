@@ -318,7 +318,8 @@ nsx_status nsx_backend_step_spmv(nsx_ctx* ctx, float* input_current);
 
 /* ---- launch-shape autotuner --------------------------------------------------------------------
  * No reference counterpart (the reference has one fixed wgpu dispatch shape, gpu_backend.rs:
- * 921-981).  Times trial_steps steps for each candidate propagation split (wavefronts sharing a
+ * 921-981).  Times trial_steps steps (rounded up to a multiple of 200, the same steps in every
+ * trial) for each candidate propagation split (wavefronts sharing a
  * neuron block's spike segment: 1, 2, 4, 8, 16) at a 50-step graph window, then each graph window
  * (10, 25, 100, 200 steps) at the best split, then binned delivery with 8192- and 16384-entry
  * tiles where the network allows it, every trial from a device-side snapshot of the mutable

@@ -93,7 +93,8 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     if (!st) st = nsx_set_prop_split(ctx, split);
     if (!st) st = nsx_set_graph_steps(ctx, graph);
     if (st) return st;
-    const uint64_t steps = graph ? (trial_steps + graph - 1) / graph * graph : trial_steps;
+    // the same number of steps in every trial: a multiple of every window tried (200)
+    const uint64_t steps = (trial_steps + 199) / 200 * 200;
     nsx_step_stats ss{};
     if ((st = nsx_step(ctx, graph ? graph : 8, &ss))) return st;  // capture / warm up
     // every trial times the same steps: back to the snapshot (the captured graph stays valid:

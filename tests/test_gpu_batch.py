@@ -1,6 +1,6 @@
 """Batched population: every network of the batch bit-identical to its own sequential run
-(raster, state, spike counts), with direct launches and with captured graphs; per-network
-statistics exact."""
+(raster, state, spike counts), with direct launches and with captured graphs, with atomic and
+binned delivery; per-network statistics exact."""
 import numpy as np
 import pytest
 
@@ -29,14 +29,17 @@ def _solo(nsx, base, net, steps, fields):
         return st, r, eng.read_state(fields), eng.read_csr()
 
 
-@pytest.mark.parametrize("graph", [0, 25])
-def test_batch_equals_sequential(nsx, graph):
+@pytest.mark.parametrize("graph,binned", [(0, False), (25, False), (25, True)])
+def test_batch_equals_sequential(nsx, graph, binned):
     base = configs.population_base(record_raster=True, raster_capacity=1 << 23)
     nets = _nets()
     steps = 300
     fields = ["v", "u", "spike_count", "last_spike_step"]
-    print(f"batch of {len(nets)} networks, {steps} steps, graph {graph}", flush=True)
+    print(f"batch of {len(nets)} networks, {steps} steps, graph {graph}, binned {binned}",
+          flush=True)
     with nsx.BatchedPopulation(base, nets) as pop:
+        if binned:  # 4096-entry tiles: the networks straddle tile boundaries
+            pop.set_delivery(nsx.abi.DELIVERY_BINNED, 0, 4096)
         rp, col, w, d = pop.read_csr()
         if graph:
             pop.set_graph_steps(graph)

@@ -204,6 +204,10 @@ nsx_status nsx_reset_state(nsx_ctx* ctx);
 nsx_status nsx_upload_state(nsx_ctx* ctx, const nsx_state_view* s);
 nsx_status nsx_read_state(nsx_ctx* ctx, nsx_state_view* s);
 nsx_status nsx_read_weights(nsx_ctx* ctx, float* weight, uint64_t m);
+/* Replace every synapse weight (CSR slot order, m = the network's synapse count); the topology,
+ * delays and state are kept and pending deferred depressions are dropped.  Genome::build_network
+ * overrides the builder's weights with the genome's this way (genome.rs:144-152). */
+nsx_status nsx_upload_weights(nsx_ctx* ctx, const float* weight, uint64_t m);
 /* Step index of the next step (net.time / dt).  Drives are keyed by (seed, neuron, step), so a
  * checkpoint restores it together with the state for an exact resume. */
 nsx_status nsx_get_step(nsx_ctx* ctx, uint64_t* step);

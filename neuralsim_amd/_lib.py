@@ -46,6 +46,7 @@ PROTOTYPES = {
     "nsx_upload_state": (_int, [_VP, C.POINTER(abi.NsxStateView)]),
     "nsx_read_state": (_int, [_VP, C.POINTER(abi.NsxStateView)]),
     "nsx_read_weights": (_int, [_VP, _P, _u64]),
+    "nsx_upload_weights": (_int, [_VP, _P, _u64]),
     "nsx_stimulate": (_int, [_VP, _u64, _f32]),
     "nsx_get_step": (_int, [_VP, C.POINTER(_u64)]),
     "nsx_set_step": (_int, [_VP, _u64]),

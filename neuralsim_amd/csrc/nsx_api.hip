@@ -613,6 +613,18 @@ nsx_status nsx_read_weights(nsx_ctx* ctx, float* weight, uint64_t m) {
   return NSX_OK;
 }
 
+nsx_status nsx_upload_weights(nsx_ctx* ctx, const float* weight, uint64_t m) {
+  if (!ctx || (!weight && m)) return set_error(ctx, NSX_E_INVALID, "NULL argument");
+  if (!ctx->has_network) return set_error(ctx, NSX_E_STATE, "no network loaded");
+  if (m != ctx->m) return set_error(ctx, NSX_E_INVALID, "weight count differs from the network's");
+  nsx_status s = flush_pending(ctx, false);  // depressions pending on the old weights
+  if (s) return s;
+  if (m)
+    NSX_HIP(ctx, hipMemcpyAsync(ctx->w, weight, m * 4, hipMemcpyHostToDevice, ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return NSX_OK;
+}
+
 nsx_status nsx_stimulate(nsx_ctx* ctx, uint64_t neuron, float current) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (neuron >= ctx->n) return NSX_OK;  // SimulationEngine::stimulate ignores out-of-range ids

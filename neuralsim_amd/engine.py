@@ -107,6 +107,11 @@ class SimulationEngine:
         self._chk(self._lib.nsx_read_weights(self._h, _ptr(w), m), "nsx_read_weights")
         return w
 
+    def upload_weights(self, weight):
+        """Replace every weight (CSR slot order); topology, delays and state are kept."""
+        w = _c(weight, np.float32)
+        self._chk(self._lib.nsx_upload_weights(self._h, _ptr(w), len(w)), "nsx_upload_weights")
+
     # ---- state ----------------------------------------------------------------------------
     def read_state(self, fields=None, ring=False):
         n = int(self.cfg.n_neurons)
@@ -366,6 +371,14 @@ class BatchedPopulation(SimulationEngine):
         sp, sq = np.empty(k, np.uint64), np.empty(k, np.uint64)
         self._chk(self._lib.nsx_batch_stats(self._h, _ptr(sp), _ptr(sq)), "nsx_batch_stats")
         return sp, sq
+
+    def net_slots(self):
+        """Per network, its [first, end) range of synapse slots in the concatenated CSR (a
+        network's rows are contiguous)."""
+        rp = self.read_csr()[0]
+        off = self.offsets.astype(np.int64)
+        return [(int(rp[off[i]]), int(rp[off[i] + self.nets[i].n_neurons]))
+                for i in range(len(self.nets))]
 
     def split(self, global_ids):
         """(network index, local index) of global neuron ids."""

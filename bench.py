@@ -22,20 +22,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def neuron_bytes(cfg, n, spikes):
+def neuron_bytes(cfg, n, spikes, ring=True):
     """Algorithmic bytes of one update launch, a lower bound: per-neuron state read and the
     writes that always happen, ring slots read, plus per-spike state (jitter c/d, count, last
     step, pre trace record, spike list).  Writes the kernel skips when the value's bits do not
     change (a ring slot that received nothing, LIF currents that stay zero, a V at rest) are not
-    counted."""
+    counted.  ring=False: no spike in the last D - 1 steps, so no slot can hold input and the
+    update does not read the ring."""
     from neuralsim_amd import abi
     if cfg.model == abi.MODEL_IZHIKEVICH:
-        per_n, per_s = 8 + 8 + 8, 8 + 8 + 4 + 16 + 4  # v, u read+write; ring read
+        per_n, per_s, ring_b = 8 + 8 + 8, 8 + 8 + 4 + 16 + 4, 8  # v, u read+write; ring read
     elif cfg.model == abi.MODEL_LIF:
-        per_n, per_s = 16 + 16 + 4, 8 + 4 + 4  # v, 2 currents, counter, 2 ring slots read; v write
+        per_n, per_s, ring_b = 16 + 16 + 4, 8 + 4 + 4, 16  # v, 2 currents, counter, 2 slots; v
     else:
-        per_n, per_s = 4 * 8 * 2 + 16, 8 + 4 + 4  # 8 state words read+write; 2 ring slots read
-    return per_n * n + per_s * spikes
+        per_n, per_s, ring_b = 4 * 8 * 2 + 16, 8 + 4 + 4, 16  # 8 state words r+w; 2 slots
+    return (per_n - (0 if ring else ring_b)) * n + per_s * spikes
 
 
 def measured_traffic(kernel):
@@ -207,7 +208,10 @@ def bench_large(ns, n=100_000_000, steps=300, drive_scale=1.0, graph=50):
         ms_u, n_u, _ = eng.kernel_time(0)
         ms_s, n_s, _ = eng.kernel_time(1)
     upd_us = ms_u / max(n_u, 1) * 1e3
-    upd_b = neuron_bytes(spec.cfg, spec.n, pst.spikes / 50.0)
+    # a silent segment (no spike in it nor in the D - 1 steps before: st covers those) skips
+    # the ring loads
+    upd_b = neuron_bytes(spec.cfg, spec.n, pst.spikes / 50.0,
+                         ring=(pst.spikes + st.spikes) > 0)
     sim_s = steps * spec.cfg.dt_ms / 1e3
     return {"neurons": spec.n, "synapses": spec.m, "drive_scale": drive_scale,
             "build_s": build_s, "steps": steps,

@@ -102,8 +102,12 @@ __global__ void k_init_state(StepParams p, DevPtrs s, int model, nsx_config cfg)
   }
 }
 
-__global__ void k_stimulate(long long* ring_slot, uint32_t i, long long q) {
+// The ring slot of step t receives q; the slot counts as live for step t (as if a spike at t-1
+// had delivered it).
+__global__ void k_stimulate(long long* ring_slot, uint32_t i, long long q, int32_t* ev_last,
+                            int32_t t) {
   atomicAdd((unsigned long long*)(ring_slot + i), (unsigned long long)q);
+  if (ev_last[0] < t - 1) ev_last[0] = t - 1;
 }
 
 static int poisson_table(double lambda, uint32_t* thr, int cap) {
@@ -269,6 +273,16 @@ void destroy_graph(nsx_ctx* ctx) {
   ctx->graph = nullptr;
 }
 
+nsx_status ring_mark(nsx_ctx* ctx, bool never) {
+  // live: a spike at t + D - 2 would keep slots t .. t + 2D - 3 live, every slot the host wrote
+  const int32_t v = never ? kNeverStep : (int32_t)ctx->t + ctx->D - 2;
+  std::vector<int32_t> h(kEvStripes * kEvStride, v);
+  NSX_HIP(ctx, hipMemcpyAsync(ctx->d.ev_last, h.data(), h.size() * 4, hipMemcpyHostToDevice,
+                              ctx->stream));
+  NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return NSX_OK;
+}
+
 }  // namespace nsx
 
 using namespace nsx;
@@ -406,6 +420,7 @@ nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out) {
   if ((s = dev_alloc(ctx, (void**)&d.step_counts, (size_t)ctx->p.step_counts_cap * 4, true)))
     goto fail;
   if ((s = dev_alloc(ctx, (void**)&d.tbase, 64, true))) goto fail;
+  if ((s = dev_alloc(ctx, (void**)&d.ev_last, kEvStripes * kEvStride * 4, true))) goto fail;
   if (cfg->record_raster) {
     if ((s = dev_alloc(ctx, (void**)&d.raster_step, cfg->raster_capacity * 4, false))) goto fail;
     if ((s = dev_alloc(ctx, (void**)&d.raster_id, cfg->raster_capacity * 4, false))) goto fail;
@@ -467,8 +482,11 @@ nsx_status nsx_reset_state(nsx_ctx* ctx) {
   ctx->hs = {};
   ctx->t = 0;
   ctx->tbase_host = 0;
+  nsx_status sr = ring_mark(ctx, true);
+  if (sr) return sr;
   return bin_clear(ctx);
 }
+
 
 nsx_status nsx_network_size(nsx_ctx* ctx, uint64_t* n, uint64_t* m) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
@@ -577,6 +595,7 @@ nsx_status nsx_upload_state(nsx_ctx* ctx, const nsx_state_view* sv) {
       (s = copy_f(ctx, d.ring, sv->ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)))
     return s;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if ((s = ring_mark(ctx, false))) return s;
   return bin_clear(ctx);  // the uploaded ring holds every pending input
 }
 
@@ -631,7 +650,8 @@ nsx_status nsx_stimulate(nsx_ctx* ctx, uint64_t neuron, float current) {
   const uint32_t slot = (uint32_t)(ctx->t % (uint64_t)ctx->D);
   long long q = (long long)llrint((double)current * ctx->p.scale_in);
   hipLaunchKernelGGL(k_stimulate, dim3(1), dim3(1), 0, ctx->stream,
-                     ctx->d.ring + (size_t)slot * ctx->ch * ctx->n, (uint32_t)neuron, q);
+                     ctx->d.ring + (size_t)slot * ctx->ch * ctx->n, (uint32_t)neuron, q,
+                     ctx->d.ev_last, (int32_t)ctx->t);
   NSX_HIP(ctx, hipGetLastError());
   return NSX_OK;
 }
@@ -663,7 +683,7 @@ nsx_status nsx_set_step(nsx_ctx* ctx, uint64_t step) {
   NSX_HIP(ctx, hipMemsetAsync(ctx->d.nspk, 0, 16, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->t = step;
-  return NSX_OK;
+  return ring_mark(ctx, false);  // the ring's slots are read at the new step indices
 }
 
 nsx_status nsx_set_prop_split(nsx_ctx* ctx, uint32_t waves_per_block) {

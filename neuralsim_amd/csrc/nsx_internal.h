@@ -116,6 +116,9 @@ struct StepParams {
 
 // per-step spike counts are added into this many stripes (one atomic per block and step)
 constexpr uint32_t kStatStripes = 16;
+// latest step with a spike, kept in kEvStripes stripes kEvStride ints apart (one 128-byte line
+// each): a ring slot can hold input only if some neuron spiked in the D - 1 steps before it
+constexpr uint32_t kEvStripes = 16, kEvStride = 32;
 
 // Device pointers used by the step kernels.
 struct DevPtrs {
@@ -135,6 +138,7 @@ struct DevPtrs {
   uint32_t* step_counts;
   uint32_t* stat_counts;  // [stat_cap][kStatStripes] spikes per step (stats on)
   double* lfp_part;       // per-block running sum of |synaptic currents| (stats on)
+  int32_t* ev_last;       // [kEvStripes * kEvStride] latest spike step per block stripe
   int32_t* tbase;
   // network
   const uint64_t* row_ptr;
@@ -278,6 +282,9 @@ hipError_t launch_update_surrogate(nsx_ctx* ctx, int32_t off);
 nsx_status bin_materialize(nsx_ctx* ctx);
 // Mark every bucket buffer empty (after the ring was overwritten).
 nsx_status bin_clear(nsx_ctx* ctx);
+// Mark the ring as possibly holding input for the next D - 1 steps (after it was written from
+// the host), or as empty (never = true, after it was cleared).
+nsx_status ring_mark(nsx_ctx* ctx, bool never);
 // (Re)allocate the bucket buffers for the current network, or release them.
 nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t tile_entries);
 // The loaded network fits binned delivery with tiles of tile_entries entries.

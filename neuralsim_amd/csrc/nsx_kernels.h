@@ -127,8 +127,20 @@ struct RingIn {
   long long q0, q1;
 };
 
+// Whether ring slot t can hold input: some neuron spiked in [t - D + 1, t - 1] (a spike's events
+// arrive 1 .. D - 1 steps later) or the host wrote the ring.  Uniform across the grid, so a step
+// of a silent network skips every ring load.
+__device__ inline bool ring_active(const StepParams& p, const DevPtrs& s, int32_t t) {
+  int32_t m = kNeverStep;
+#pragma unroll
+  for (uint32_t k = 0; k < kEvStripes; ++k) m = max(m, s.ev_last[k * kEvStride]);
+  return m >= t - (int32_t)p.D + 1;
+}
+
 template <bool TWO>
-__device__ inline RingIn ring_load(const StepParams& p, const DevPtrs& s, uint32_t i, int32_t t) {
+__device__ inline RingIn ring_load(const StepParams& p, const DevPtrs& s, uint32_t i, int32_t t,
+                                   bool active) {
+  if (!active) return RingIn{nullptr, 0, 0};
   const uint32_t slot = (uint32_t)t % p.D;
   RingIn r;
   r.r0 = s.ring + (size_t)slot * p.ch * p.n + i;
@@ -216,6 +228,7 @@ __device__ inline void block_compact(const StepParams& p, const DevPtrs& s, bool
   if (threadIdx.x == 0) {
     s.bcnt[blk] = tot;
     if (tot) s.part[kPartSpikes * p.nblk + blk] += tot;
+    if (tot) s.ev_last[(blk % kEvStripes) * kEvStride] = t;
     if (p.stats_on && tot)
       atomicAdd(&s.stat_counts[((uint32_t)t % p.stat_cap) * kStatStripes + blk % kStatStripes],
                 tot);

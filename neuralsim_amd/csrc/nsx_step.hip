@@ -46,8 +46,9 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
   HHSyn y{0.0f, 0.0f, 0.0f, 0.0f};
   // blocks with no spike at t-1 (their segment count is 0) skip the last-spike loads
   const uint32_t prev = PAIR ? s.bcnt[blockIdx.x] : 0u;
+  const bool ring_in = ring_active(p, s, t);
   if (live) {
-    rg = ring_load<TWO>(p, s, i, t);
+    rg = ring_load<TWO>(p, s, i, t, ring_in);
     v = s.v[i];
     if (PAIR && prev) last = s.last[i];
     if (MODEL == NSX_MODEL_LIF) {

@@ -411,8 +411,9 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
   float x[kMlpIn];
   int32_t last = kNeverStep;
   const uint32_t prev = PAIR ? s.bcnt[blockIdx.x] : 0u;  // 0: no spike at t-1 in the block
+  const bool ring_in = ring_active(p, s, t);
   if (live) {
-    rg = ring_load<true>(p, s, i, t);
+    rg = ring_load<true>(p, s, i, t, ring_in);
     if (PAIR && prev) last = s.last[i];
     x[0] = s.v[i];
     x[1] = s.gm[i];

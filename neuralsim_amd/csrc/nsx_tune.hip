@@ -33,7 +33,7 @@ static std::vector<Buf> mutable_buffers(nsx_ctx* ctx) {
       {d.spikes, nblk * kBlock * 4},          {d.bcnt, nblk * 4},
       {d.part, nblk * kNumPart * 8},          {d.counters, kNumCounters * 8},
       {d.step_counts, (size_t)ctx->p.step_counts_cap * 4},
-      {d.tbase, 64}};
+      {d.tbase, 64},          {d.ev_last, kEvStripes * kEvStride * 4}};
   if (ctx->cfg.plasticity != NSX_PLAST_NONE) b.push_back({ctx->w, ctx->m * 4});
   if (ctx->sur.fb) b.push_back({ctx->sur.fb, nblk * 8});
   return b;

@@ -206,6 +206,35 @@ def test_determinism_and_stimulate(nsx, orc):
     assert 17 in ids[st_ == 40] or 17 in ids[st_ == 41]
 
 
+def test_silent_network_then_stimulus(nsx, orc):
+    """A network with no spikes skips its ring loads (no slot can hold input); a stimulus makes
+    the slot live again and the spikes it causes propagate: state and ring equal the mirror's."""
+    spec = nsx.configs.lif_network(n=3000, seed=4, drive_scale=0.0, record_raster=True,
+                                   raster_capacity=1 << 20, k_in=100)
+    fields = ["v", "syn0", "syn1", "refractory"]
+    print("case silent then stimulus", flush=True)
+    with nsx.SimulationEngine.from_spec(spec) as eng:
+        eng.set_graph_steps(25)
+        eng.step(100)
+        assert eng.raster()[0].size == 0
+        for k in range(40):
+            eng.stimulate(k, 400.0)
+        eng.step(200)
+        g_steps, g_ids = eng.raster()
+        g = eng.read_state(fields, ring=True)
+    mir = orc.MirrorNet.from_spec(spec)
+    mir.run(100)
+    for k in range(40):
+        mir.stimulate(k, 400.0)
+    m_steps, m_ids, _ = mir.run(200)
+    m = mir.read_state(fields, ring=True)
+    assert len(m_steps) >= 40
+    assert np.any(m["syn0"][40:] != 0)  # the stimulated spikes reached other neurons
+    assert np.array_equal(g_steps, m_steps) and np.array_equal(g_ids, m_ids)
+    for k in m:
+        assert np.array_equal(_bits(g[k]), _bits(m[k])), k
+
+
 def test_checkpoint_roundtrip(nsx, orc):
     """read_state/read_csr/get_step -> fresh engine continues bit-identically."""
     spec = nsx.configs.izh_stdp_network(n=2000, seed=12, record_raster=True,

@@ -155,6 +155,16 @@ def test_network_replacement_keeps_pending_input(nsx):
     _same(out[0], out[1])
 
 
+def test_binned_delivery_edge_networks(nsx):
+    """No synapses at all, and a network whose every step overflows a one-event buffer."""
+    empty = configs.lif_network(n=3000, seed=6, drive_scale=60.0, k_in=0, record_raster=True,
+                                raster_capacity=1 << 20)
+    print("case delivery edge: empty network, one-event buffer", flush=True)
+    _same(_run(nsx, empty, 300, mode=ATOMIC), _run(nsx, empty, 300, graph=50))
+    spec = configs.izh_stdp_network(n=4000, seed=2, record_raster=True, raster_capacity=1 << 22)
+    _same(_run(nsx, spec, 100, mode=ATOMIC), _run(nsx, spec, 100, cap=1, te=4096, graph=20))
+
+
 def test_binned_delivery_rejects_unsupported_shapes(nsx):
     spec = configs.izh_stdp_network(n=3000, seed=4)
     with nsx.SimulationEngine.from_spec(spec) as eng:

@@ -47,24 +47,36 @@ __device__ inline void gen_synapse(uint64_t seed, uint32_t n, uint32_t n_exc, fl
   delay = (uint8_t)(d_lo + (uint32_t)(((uint64_t)r.z * d_span) >> 32));
 }
 
-// RNG key, local index and network size of neuron i of block blk (see NeuronId).
+// RNG key, local index and network size of neuron i of block blk (see NeuronId).  The table
+// entry is read with relaxed atomic loads: plain loads here and the parameter block's fields
+// in the other branch get merged into one load through a selected pointer, a vector load the
+// kernel then has to wait for.
 __device__ inline NeuronId neuron_id(const StepParams& p, const DevPtrs& s, uint32_t i,
                                      uint32_t blk) {
   if (!p.batch) return NeuronId{p.seed, i, p.n, p.n_exc};
-  const BatchNet& b = s.bnet[s.blk_net[blk]];
-  return NeuronId{b.seed, i - b.offset, b.n, b.n_exc};
+  const BatchNet* b = s.bnet + s.blk_net[blk];
+  constexpr int kWg = __HIP_MEMORY_SCOPE_WORKGROUP;
+  const uint64_t seed = __hip_atomic_load(&b->seed, __ATOMIC_RELAXED, kWg);
+  const uint32_t off = __hip_atomic_load(&b->offset, __ATOMIC_RELAXED, kWg);
+  const uint32_t n = __hip_atomic_load(&b->n, __ATOMIC_RELAXED, kWg);
+  const uint32_t ne = __hip_atomic_load(&b->n_exc, __ATOMIC_RELAXED, kWg);
+  return NeuronId{seed, i - off, n, ne};
 }
 
 // Drive words of one step for a block's neurons.  One Philox draw covers four consecutive
 // neurons of a network (local indices 4g .. 4g+3; blocks start at a multiple of 256 of their
 // network): wavefront 0 draws the block's 64 groups into w[0 .. kBlock).  The caller synchronises
-// the block before drive_draw reads them.
+// the block before drive_draw reads them.  `id` is the calling thread's NeuronId, computed
+// before the neuron's loads were issued: looking the network's key up here again would be a
+// load after them, and waiting for it (memory counters retire in order) would wait for all of
+// them before the draws could start.
 template <int DRIVE>
-__device__ inline void drive_stage(const StepParams& p, const DevPtrs& s, int32_t t, uint32_t* w) {
+__device__ inline void drive_stage(const StepParams& p, const DevPtrs& s, int32_t t, uint32_t* w,
+                                   const NeuronId& id) {
   if (DRIVE == NSX_DRIVE_NONE) return;
   if (threadIdx.x < kBlock / 4) {
-    const NeuronId id0 = neuron_id(p, s, blockIdx.x * kBlock, blockIdx.x);
-    const U4 r = draw(id0.seed, (uint64_t)(id0.local >> 2) + threadIdx.x, (uint32_t)t,
+    const uint32_t local0 = id.local - threadIdx.x;  // the block's first neuron
+    const U4 r = draw(id.seed, (uint64_t)(local0 >> 2) + threadIdx.x, (uint32_t)t,
                       kStreamDrive);
     w[4 * threadIdx.x] = r.x;
     w[4 * threadIdx.x + 1] = r.y;
@@ -83,9 +95,14 @@ __device__ inline void drive_draw(const StepParams& p, const uint32_t* w, float&
   if (DRIVE == NSX_DRIVE_NONE) return;
   const uint32_t k = threadIdx.x;
   if (DRIVE == NSX_DRIVE_POISSON) {
+    // the first four thresholds at constant offsets (fetched by scalar loads up front); the
+    // rest only in the rare wavefront where some neuron's word passes the fourth (k >= 4)
     const uint32_t x = w[k];
     uint32_t c = 0;
-    for (int q = 0; q < p.npthr; ++q) c += (x >= p.pthr[q]) ? 1u : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c += (q < p.npthr && x >= p.pthr[q]) ? 1u : 0u;
+    if (p.npthr > 4 && x >= p.pthr[3])
+      for (int q = 4; q < p.npthr; ++q) c += (x >= p.pthr[q]) ? 1u : 0u;
     float f = (float)c * p.drive_weight;
     a0 = a0 + f;
   } else {

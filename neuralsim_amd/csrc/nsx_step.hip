@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
       y = hh_syn_load<DRIVE>(s, i);
     }
   }
-  drive_stage<DRIVE>(p, s, t, s_drive);
+  drive_stage<DRIVE>(p, s, t, s_drive, id);
   if (DRIVE != NSX_DRIVE_NONE) __syncthreads();  // the staged words
   bool spiked = false;
   float lfp = 0.0f;  // HH: this neuron's |synaptic currents|

@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
     y = hh_syn_load<DRIVE>(s, i);
   }
   stage_scalars(S, q.img, q.norm);
-  drive_stage<DRIVE>(p, s, t, s_drive);
+  drive_stage<DRIVE>(p, s, t, s_drive, id);
   __syncthreads();
   const MlpNorm& nm = S.norm;
   float lfp = 0.0f;

@@ -41,7 +41,8 @@ struct __attribute__((aligned(16))) TraceRec {
 constexpr uint32_t kPostPend = 4;
 
 // Post-synaptic record, 32 bytes: the lazily decayed trace {v, s}, the step of the neuron's last
-// spike (same-step post spikes seen by the potentiation pass), and the pending post spikes.  The trace bump of a spike at step t is applied by the update kernel of step t+1
+// spike (same-step post spikes seen by the potentiation pass), and the pending post spikes.  The
+// trace bump of a spike at step t is applied by the update kernel of step t+1
 // (after every pass of step t has read the pre-bump value), which also empties a list that the
 // synapse pass of step t flushed.  Triplet STDP keeps no trace: s holds the step of the
 // neuron's spike before `last`.

@@ -223,6 +223,9 @@ nsx_status nsx_simulate_ms(nsx_ctx* ctx, double duration_ms, nsx_step_stats* sta
 nsx_status nsx_set_prop_split(nsx_ctx* ctx, uint32_t waves_per_block);
 /* Use a captured hipGraph of `steps_per_graph` steps for nsx_step (0 = direct launches). */
 nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph);
+/* Neurons per thread of the LIF / Izhikevich update: 1, 4, or 0 = by network size (4 from 2^21
+ * neurons).  Results do not depend on it; HH always runs one per thread. */
+nsx_status nsx_set_update_width(nsx_ctx* ctx, uint32_t neurons_per_thread);
 
 /* How propagated events reach their ring slot (the reference adds each event into
  * pending_input, engine.rs:414-494).  Results are bit-identical in both modes: the arrivals are

@@ -695,6 +695,15 @@ nsx_status nsx_set_prop_split(nsx_ctx* ctx, uint32_t waves_per_block) {
   return NSX_OK;
 }
 
+nsx_status nsx_set_update_width(nsx_ctx* ctx, uint32_t neurons_per_thread) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (neurons_per_thread != 0 && neurons_per_thread != 1 && neurons_per_thread != 4)
+    return set_error(ctx, NSX_E_INVALID, "neurons_per_thread must be 0, 1 or 4");
+  ctx->update_width = neurons_per_thread;
+  destroy_graph(ctx);  // captured launches hold the old kernel
+  return NSX_OK;
+}
+
 nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (steps_per_graph > 4096) return set_error(ctx, NSX_E_INVALID, "steps_per_graph > 4096");

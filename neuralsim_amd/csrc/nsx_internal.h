@@ -253,6 +253,7 @@ struct nsx_ctx {
   uint64_t bin_cap_req = 0;
   uint32_t bin_te_req = 0;  // tile entries requested (0: automatic)
   uint32_t bin_te = 0;      // tile entries in use
+  uint32_t update_width = 0;  // neurons per thread of the point-model update (0: by size)
   bool bin_dirty = false;  // the network changed: buffers are re-sized before the next step
   // SimulationStats kept on the host from the device's per-step counts (nsx_set_stats)
   struct {

@@ -176,6 +176,11 @@ class SimulationEngine:
         self._chk(self._lib.nsx_set_prop_split(self._h, int(waves_per_block)),
                   "nsx_set_prop_split")
 
+    def set_update_width(self, neurons_per_thread):
+        """Neurons per thread of the LIF / Izhikevich update (1, 4; 0 = by network size)."""
+        self._chk(self._lib.nsx_set_update_width(self._h, int(neurons_per_thread)),
+                  "nsx_set_update_width")
+
     def set_delivery(self, mode, cap_events=0, tile_entries=0):
         """How events reach the ring (abi.DELIVERY_ATOMIC / DELIVERY_BINNED); results are
         bit-identical.  cap_events bounds one step's bucket buffer (more events: atomics that

@@ -198,6 +198,39 @@ __device__ inline void ring4(const long long* r, uint32_t i0, uint32_t n, bool a
   }
 }
 
+// Write back four consecutive values when any of them changed (bit mask `chg`): one 16-byte
+// store when all four lie inside the array, else element stores of the changed ones.  Silent
+// neurons cost no write; active ones a coalesced one.
+template <typename T>
+__device__ inline void store4(T* a, uint32_t i0, uint32_t n, const T v[4], uint32_t chg) {
+  static_assert(sizeof(T) == 4, "4-byte elements");
+  if (!chg) return;
+  if (i0 + 4 <= n) {
+    *reinterpret_cast<uint4*>(a + i0) =
+        make_uint4(__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
+                   __builtin_bit_cast(uint32_t, v[2]), __builtin_bit_cast(uint32_t, v[3]));
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      if (chg & (1u << k)) a[i0 + k] = v[k];
+  }
+}
+
+// Clear four ring entries of one plane after they were read (nz: bit k set when entry k held
+// input).
+__device__ inline void ring_clear4(long long* r, uint32_t i0, uint32_t n, bool aligned,
+                                   uint32_t nz) {
+  if (!nz) return;
+  if (aligned && i0 + 4 <= n) {
+    *reinterpret_cast<longlong2*>(r + i0) = make_longlong2(0, 0);
+    *reinterpret_cast<longlong2*>(r + i0 + 2) = make_longlong2(0, 0);
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+      if (nz & (1u << k)) r[i0 + k] = 0;
+  }
+}
+
 template <int MODEL, int DRIVE, int PL>
 __global__ __launch_bounds__(kBlock) void k_update4(StepParams p, DevPtrs s, int32_t off) {
   static_assert(MODEL == NSX_MODEL_LIF || MODEL == NSX_MODEL_IZHIKEVICH, "point models");
@@ -238,16 +271,22 @@ __global__ __launch_bounds__(kBlock) void k_update4(StepParams p, DevPtrs s, int
     dr = draw(id.seed, (uint64_t)(id.local >> 2), (uint32_t)t, kStreamDrive);
   const uint32_t words[4] = {dr.x, dr.y, dr.z, dr.w};
   uint32_t spk = 0;  // bit k: neuron i0 + k spiked
+  uint32_t nz0 = 0, nz1 = 0;  // ring entries that held input (cleared below)
+  // LIF: the new currents, counter and V, and which of them changed
+  float ie4[4] = {w1[0], w1[1], w1[2], w1[3]}, ii4[4] = {w2[0], w2[1], w2[2], w2[3]};
+  float vn4[4] = {v[0], v[1], v[2], v[3]};
+  int32_t rf4[4] = {refr[0], refr[1], refr[2], refr[3]};
+  uint32_t c_ie = 0, c_ii = 0, c_v = 0, c_rf = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 4; ++k) {
     if (!live[k]) continue;
     const uint32_t i = i0 + k;
     post_bump_prev<PAIR>(p, s, i, last[k], t);
     float a0 = dequantize(q0[k], p.scale_out), a1 = 0.0f;
-    if (q0[k] != 0) r0[i] = 0;
+    if (q0[k] != 0) nz0 |= 1u << k;
     if (TWO) {
       a1 = dequantize(q1[k], p.scale_out);
-      if (q1[k] != 0) r0[p.n + i] = 0;
+      if (q1[k] != 0) nz1 |= 1u << k;
     }
     float z = 0.0f;
     if (DRIVE == NSX_DRIVE_POISSON) {
@@ -269,10 +308,13 @@ __global__ __launch_bounds__(kBlock) void k_update4(StepParams p, DevPtrs s, int
       ie = ie + a0;
       float ii = w2[k] * p.dec_i;
       ii = ii + a1;
-      if (__float_as_uint(ie) != __float_as_uint(w1[k])) s.s0[i] = ie;
-      if (__float_as_uint(ii) != __float_as_uint(w2[k])) s.s1[i] = ii;
+      ie4[k] = ie;
+      ii4[k] = ii;
+      if (__float_as_uint(ie) != __float_as_uint(w1[k])) c_ie |= 1u << k;
+      if (__float_as_uint(ii) != __float_as_uint(w2[k])) c_ii |= 1u << k;
       if (refr[k] > 0) {
-        s.refr[i] = refr[k] - 1;
+        rf4[k] = refr[k] - 1;
+        c_rf |= 1u << k;
       } else {
         float I = ie + ii;
         float x = v[k] - p.v_rest;
@@ -283,10 +325,12 @@ __global__ __launch_bounds__(kBlock) void k_update4(StepParams p, DevPtrs s, int
         float vn = v[k] + x;
         if (vn >= p.v_th) {
           vn = p.v_reset;
-          s.refr[i] = p.refr_steps;
+          rf4[k] = p.refr_steps;
+          c_rf |= 1u << k;
           spiked = true;
         }
-        if (__float_as_uint(vn) != __float_as_uint(v[k])) s.v[i] = vn;
+        vn4[k] = vn;
+        if (__float_as_uint(vn) != __float_as_uint(v[k])) c_v |= 1u << k;
       }
     } else {
       float I = a0;
@@ -328,6 +372,14 @@ __global__ __launch_bounds__(kBlock) void k_update4(StepParams p, DevPtrs s, int
       spike_record<PL>(p, s, i, t);
       spk |= 1u << k;
     }
+  }
+  ring_clear4(r0, i0, p.n, ring_al, nz0);
+  if (TWO) ring_clear4(r0 + p.n, i0, p.n, ring_al, nz1);
+  if (MODEL == NSX_MODEL_LIF) {
+    store4(s.s0, i0, p.n, ie4, c_ie);
+    store4(s.s1, i0, p.n, ii4, c_ii);
+    store4(s.refr, i0, p.n, rf4, c_rf);
+    store4(s.v, i0, p.n, vn4, c_v);
   }
   if (MODEL == NSX_MODEL_IZHIKEVICH) {  // v and u are always written
     if (i0 + 4 <= p.n && live[3]) {

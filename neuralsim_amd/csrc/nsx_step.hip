@@ -455,7 +455,17 @@ struct RowChunk {
   int32_t pre_sp[64];
   uint32_t it_end[64];   // batched rows: inclusive prefix of the items' row counts
   uint32_t it_base[64];  // and each item's first spike-segment index
+  uint32_t chan[64];     // ring channel of the row's input (two-channel models)
 };
+
+// Ring channel of source i's synapses: 1 for an inhibitory source (outside [0, n_exc) of its
+// own network) in a two-channel model, else 0.
+__device__ inline uint32_t source_channel(const StepParams& p, const DevPtrs& s, uint32_t i) {
+  if (p.ch < 2) return 0u;
+  if (!p.batch) return i >= p.n_exc ? 1u : 0u;
+  const BatchNet& b = s.bnet[s.blk_net[i / kBlock]];
+  return i - b.offset >= b.n_exc ? 1u : 0u;
+}
 
 // Inclusive wave scan of deg; stores prefix / start / source of each lane's row in LDS and
 // returns the total.
@@ -536,14 +546,11 @@ __device__ inline float pair_flush(const StepParams& p, const DevPtrs& s, float 
   return w;
 }
 
-// Propagation work item: part q of neuron block b.  PL: STDP applied on the spiking rows' E->E
-// slots -- pair: the target's pending depressions, potentiation by the post trace (then
-// depression if the target fired at t); triplet: depression by the target's last spike.
 // Binned delivery of one event: its slot in the step's bucket buffer comes from the
 // workgroup's LDS cursor of bucket (d - 1, target tile); the record is the target's LDS entry
 // in its tile's accumulator (channel-major) and the delivered weight.
 // A slot that does not exist (live = false) takes no place: its cursor add is 0 and nothing is
-// stored, but its loads stay unconditional so that both slots' loads are in flight together.
+// stored, but its loads stay unconditional so that all slots' loads are in flight together.
 __device__ inline void deliver_bin(const StepParams& p, uint32_t* pos, uint2* ev, uint32_t d,
                                    uint32_t j, uint32_t c, float w, bool live) {
   const uint32_t bin = (d - 1u) * p.bin_ntiles + (j >> p.bin_tshift);
@@ -553,89 +560,99 @@ __device__ inline void deliver_bin(const StepParams& p, uint32_t* pos, uint2* ev
 }
 
 // The slots of one row chunk (chunk_setup done): propagation and the STDP at the pre spike.
-// Returns the potentiated / depressed-at-pre slot count of this lane.
+// Each lane takes kPropSlots slots per round, 64 apart, and issues all their loads before the
+// first atomic.  A slot past the chunk's end repeats the previous slot's loads (in bounds) and
+// stores nothing.  Returns the potentiated / depressed-at-pre slot count of this lane.
+constexpr uint32_t kPropSlots = 2;
+
 template <int PL, bool BIN>
 __device__ unsigned long long propagate_chunk(const StepParams& p, const DevPtrs& s,
                                               const RowChunk& rc, int32_t t, uint32_t total,
                                               uint32_t lane, uint32_t* pos, uint2* bev) {
+  constexpr uint32_t NS = kPropSlots;
   const uint32_t tm = (uint32_t)t % p.D;
   const size_t plane = (size_t)p.ch * p.n;
   unsigned long long ltp = 0;
   uint32_t r = 0;
-  for (uint32_t f = lane; f < total; f += 128) {
-    // two slots per lane: both sets of loads are issued before the atomics
-    const bool two = f + 64 < total;
-    const uint64_t k0 = chunk_slot(rc, f, r);
-    const uint32_t src0 = rc.src[r];
-    uint32_t r1 = r;
-    const uint64_t k1 = two ? chunk_slot(rc, f + 64, r1) : k0;
-    const uint32_t src1 = rc.src[r1];
-    const uint32_t j0 = s.col[k0], j1 = s.col[k1];
-    float w0 = s.w[k0], w1 = s.w[k1];
-    const uint32_t d0 = s.delay[k0], d1 = s.delay[k1];
-    const bool pot0 = PL != NSX_PLAST_NONE && src0 < p.n_exc && j0 < p.n_exc;
-    const bool pot1 = PL != NSX_PLAST_NONE && two && src1 < p.n_exc && j1 < p.n_exc;
-    PostRec pr0{0.0f, kNeverStep, kNeverStep, 0u, {}}, pr1{0.0f, kNeverStep, kNeverStep, 0u, {}};
-    if (pot0) pr0 = s.post[j0];  // 32 bytes: trace, last spike, pending spikes
-    if (pot1) pr1 = s.post[j1];
-    TraceRec a0{}, a1{};
+  for (uint32_t f = lane; f < total; f += 64 * NS) {
+    bool has[NS], pot[NS];
+    uint64_t k[NS];
+    uint32_t rr[NS], src[NS], j[NS], d[NS];
+    float w[NS];
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) {
+      has[q] = q == 0 || f + 64 * q < total;
+      uint32_t rq = q ? rr[q - 1] : r;
+      k[q] = has[q] ? chunk_slot(rc, f + 64 * q, rq) : k[q - 1];
+      rr[q] = rq;
+      src[q] = rc.src[rq];
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) j[q] = s.col[k[q]];
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) w[q] = s.w[k[q]];
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) d[q] = s.delay[k[q]];
+    PostRec pr[NS];
+    TraceRec a[NS];
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) {
+      pot[q] = PL != NSX_PLAST_NONE && has[q] && src[q] < p.n_exc && j[q] < p.n_exc;
+      pr[q] = PostRec{0.0f, kNeverStep, kNeverStep, 0u, {}};
+      if (pot[q]) pr[q] = s.post[j[q]];  // 32 bytes: trace, last spike, pending spikes
+      a[q] = TraceRec{};
+    }
     if (PL == NSX_PLAST_PAIR) {  // the target's pending depressions precede the delivery
-      if (pot0) {
-        a0 = TraceRec{rc.pre_v[r], 0, rc.pre_vp[r], rc.pre_sp[r]};
-        w0 = pair_pending(p, s, w0, pr0, a0, t);
-      }
-      if (pot1) {
-        a1 = TraceRec{rc.pre_v[r1], 0, rc.pre_vp[r1], rc.pre_sp[r1]};
-        w1 = pair_pending(p, s, w1, pr1, a1, t);
+#pragma unroll
+      for (uint32_t q = 0; q < NS; ++q) {
+        if (pot[q]) {
+          a[q] = TraceRec{rc.pre_v[rr[q]], 0, rc.pre_vp[rr[q]], rc.pre_sp[rr[q]]};
+          w[q] = pair_pending(p, s, w[q], pr[q], a[q], t);
+        }
       }
     }
-    uint32_t sl0 = tm + d0, sl1 = tm + d1;
-    sl0 = sl0 >= p.D ? sl0 - p.D : sl0;
-    sl1 = sl1 >= p.D ? sl1 - p.D : sl1;
-    const uint32_t ch0 = (p.ch > 1 && src0 >= p.n_exc) ? 1u : 0u;
-    const uint32_t ch1 = (p.ch > 1 && src1 >= p.n_exc) ? 1u : 0u;
-    if (BIN && pos) {
-      deliver_bin(p, pos, bev, d0, j0, ch0, w0, true);
-      deliver_bin(p, pos, bev, d1, j1, ch1, w1, two);
-    } else {
-      atomicAdd((unsigned long long*)(s.ring + sl0 * plane + (size_t)ch0 * p.n + j0),
-                (unsigned long long)quantize(w0, p.scale_in));
-      if (two)
-        atomicAdd((unsigned long long*)(s.ring + sl1 * plane + (size_t)ch1 * p.n + j1),
-                  (unsigned long long)quantize(w1, p.scale_in));
+#pragma unroll
+    for (uint32_t q = 0; q < NS; ++q) {
+      uint32_t sl = tm + d[q];
+      sl = sl >= p.D ? sl - p.D : sl;
+      const uint32_t ch = p.ch > 1 ? rc.chan[rr[q]] : 0u;
+      if (BIN && pos) {
+        deliver_bin(p, pos, bev, d[q], j[q], ch, w[q], has[q]);
+      } else if (has[q]) {
+        atomicAdd((unsigned long long*)(s.ring + sl * plane + (size_t)ch * p.n + j[q]),
+                  (unsigned long long)quantize(w[q], p.scale_in));
+      }
     }
-    if (PL == NSX_PLAST_TRIPLET && (pot0 || pot1)) {
+    if (PL == NSX_PLAST_TRIPLET) {
       // pre spike at t: depression by the target's last spike (same step included)
-      if (pot0) {
-        const float x = trace_at(s.mtab, p.mtab_n, t - pr0.last);
-        const float dlt = p.a_minus * x;
-        const float wn = w0 - dlt;
-        s.w[k0] = clampf_(wn, p.w_min, p.w_max);
-        ltp += 1;
-      }
-      if (pot1) {
-        const float x = trace_at(s.mtab, p.mtab_n, t - pr1.last);
-        const float dlt = p.a_minus * x;
-        const float wn = w1 - dlt;
-        s.w[k1] = clampf_(wn, p.w_min, p.w_max);
-        ltp += 1;
+#pragma unroll
+      for (uint32_t q = 0; q < NS; ++q) {
+        if (pot[q]) {
+          const float x = trace_at(s.mtab, p.mtab_n, t - pr[q].last);
+          const float dlt = p.a_minus * x;
+          const float wn = w[q] - dlt;
+          s.w[k[q]] = clampf_(wn, p.w_min, p.w_max);
+          ltp += 1;
+        }
       }
     }
-    if (PL == NSX_PLAST_PAIR && (pot0 || pot1)) {
-      if (pot0) {
-        s.w[k0] = pair_at_pre(p, s, w0, pr0, a0, t);
-        ltp += 1;
-      }
-      if (pot1) {
-        s.w[k1] = pair_at_pre(p, s, w1, pr1, a1, t);
-        ltp += 1;
+    if (PL == NSX_PLAST_PAIR) {
+#pragma unroll
+      for (uint32_t q = 0; q < NS; ++q) {
+        if (pot[q]) {
+          s.w[k[q]] = pair_at_pre(p, s, w[q], pr[q], a[q], t);
+          ltp += 1;
+        }
       }
     }
-    r = r1;
+    r = rr[NS - 1];
   }
   return ltp;
 }
 
+// Propagation work item: part q of neuron block b.  PL: STDP applied on the spiking rows' E->E
+// slots -- pair: the target's pending depressions, potentiation by the post trace (then
+// depression if the target fired at t); triplet: depression by the target's last spike.
 template <int PL, bool BIN>
 __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
                                uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane,
@@ -653,6 +670,7 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
       beg = s.row_ptr[i];
       deg = (uint32_t)(s.row_ptr[i + 1] - beg);
     }
+    if (p.ch > 1 && lane < nr) rc.chan[lane] = source_channel(p, s, i);
     if (PL == NSX_PLAST_PAIR && lane < nr && i < p.n_exc) {
       const TraceRec a = s.pre[i];
       rc.pre_v[lane] = a.v;
@@ -955,6 +973,7 @@ __device__ inline unsigned long long batch_rows(const StepParams& p, const DevPt
       i = s.spikes[rc.it_base[l] + k * P];
       beg = s.row_ptr[i];
       deg = (uint32_t)(s.row_ptr[i + 1] - beg);
+      if (p.ch > 1) rc.chan[lane] = source_channel(p, s, i);
       if (pre_rec && i < p.n_exc) {
         const TraceRec a = s.pre[i];
         rc.pre_v[lane] = a.v;

@@ -81,3 +81,48 @@ def test_population_evaluate(nsx):
     assert res["steps"] == 200 and len(res["fitness"]) == len(nets)
     assert np.all(res["fitness"] > 0) and np.all(res["fitness"] <= 1)
     assert np.all(res["mean_rate"] >= 0)
+
+
+def _model_case(model):
+    """Base config and networks of a two-channel batch: the LIF and HH workloads' neuron,
+    drive and synapse settings, seven networks of ragged sizes."""
+    if model == "lif":
+        base = configs.lif_network(n=2, seed=1, drive_scale=60.0, record_raster=True,
+                                   raster_capacity=1 << 22).cfg
+        nets = configs.population_nets(5, seed=23, n_range=(300, 3000), k_range=(10, 120),
+                                       w_exc=(0.0, 0.5), w_inh=(-2.0, 0.0), delays=(10, 50))
+        return base, nets, 1500, ["v", "syn0", "syn1", "refractory", "spike_count"]
+    base = configs.hh_network(n=2, seed=8, record_raster=True, raster_capacity=1 << 20).cfg
+    nets = configs.population_nets(4, seed=29, n_range=(300, 1500), k_range=(20, 100),
+                                   w_exc=(0.0, 0.004), w_inh=(-0.02, 0.0), delays=(100, 200))
+    return base, nets, 3000, ["v", "m", "h", "n", "syn0", "syn1", "syn2", "spike_count"]
+
+
+@pytest.mark.parametrize("model,binned", [("lif", False), ("lif", True), ("hh", False)])
+def test_two_channel_batch_equals_sequential(nsx, model, binned):
+    """Inhibitory sources feed the second ring channel by their own network's E/I split."""
+    base, nets, steps, fields = _model_case(model)
+    print(f"two-channel batch {model}: {len(nets)} networks, {steps} steps, binned {binned}",
+          flush=True)
+    with nsx.BatchedPopulation(base, nets) as pop:
+        if binned:
+            pop.set_delivery(nsx.abi.DELIVERY_BINNED, 0, 4096)
+        pop.set_graph_steps(50)
+        pst = pop.step(steps)
+        b_steps, b_ids = pop.raster()
+        b_state = pop.read_state(fields)
+        off = pop.offsets.astype(np.int64)
+        net_of, local = pop.split(b_ids)
+    total = 0
+    for i, net in enumerate(nets):
+        st, (s_steps, s_ids), s_state, _ = _solo(nsx, base, net, steps, fields)
+        a, n = off[i], net.n_neurons
+        sel = net_of == i
+        assert np.array_equal(b_steps[sel], s_steps) and np.array_equal(local[sel], s_ids), i
+        for f in fields:
+            assert np.array_equal(b_state[f][a:a + n].view(np.uint32),
+                                  s_state[f].view(np.uint32)), (i, f)
+        total += st.total_spikes
+    assert pst.total_spikes == total > 100
+    # the inhibitory channel carried input
+    assert np.any(b_state["syn1"] != 0)

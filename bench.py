@@ -234,7 +234,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2000,
                     help="steps of the direct-launch segment timed per kernel")
     ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=30,
+                    help="steps of the CPU baseline (about 0.5 s each on one core)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hh", action="store_true", help="skip the HH surrogate comparison")
     ap.add_argument("--no-pop", action="store_true", help="skip the batched population")

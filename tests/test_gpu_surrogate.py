@@ -208,3 +208,44 @@ def test_spike_displacement_and_gain(nsx, orc, trained):
                                                            1e-9)
     assert gain >= 5.0
     assert res["euler_0.05"]["displaced_fraction"] >= 5 * res["surrogate"]["displaced_fraction"]
+
+
+@pytest.mark.parametrize("plastic", [False, True])
+def test_coupled_surrogate_network_same_in_every_delivery(nsx, trained, plastic):
+    """A coupled 3000-neuron surrogate network (AMPA/GABA-A/NMDA input through the delay ring,
+    optionally pair STDP): direct launches with atomic delivery, graph windows with atomic
+    delivery and graph windows with binned delivery give the same raster, state, ring and
+    weights bit for bit."""
+    _, params, norm, _ = trained
+    spec = configs.hh_surrogate_network(n=3000, seed=13, record_raster=True,
+                                        raster_capacity=1 << 22, w_exc=(0.0, 0.02),
+                                        w_inh=(-0.05, 0.0))
+    if plastic:
+        spec.cfg.plasticity = abi.PLAST_PAIR
+        spec.cfg.w_min, spec.cfg.w_max = 0.0, 0.05
+    fields = ["v", "m", "h", "n", "syn0", "syn1", "syn2", "drive", "spike_count"]
+    out = []
+    for mode, graph, te in [(abi.DELIVERY_ATOMIC, 0, 0), (abi.DELIVERY_ATOMIC, 50, 0),
+                            (abi.DELIVERY_BINNED, 50, 4096)]:
+        print(f"case coupled surrogate plastic={plastic} mode={mode} graph={graph}", flush=True)
+        with nsx.SimulationEngine.from_spec(spec) as eng:
+            eng.set_surrogate_params(params, norm)
+            eng.set_delivery(mode, 0, te)
+            eng.set_graph_steps(graph)
+            st = eng.step(700)
+            st = eng.step(1300)
+            out.append((st, eng.raster(), eng.weights(), eng.read_state(fields, ring=True)))
+    (st0, (s0, i0), w0, x0) = out[0]
+    assert st0.total_spikes > 3000 and st0.events > 100_000
+    assert np.any(x0["syn1"] != 0)  # inhibitory input arrived
+    for st, (s, i), w, x in out[1:]:
+        assert st.total_spikes == st0.total_spikes and st.total_events == st0.total_events
+        assert np.array_equal(s, s0) and np.array_equal(i, i0), "raster differs"
+        assert np.array_equal(w.view(np.uint32), w0.view(np.uint32)), "weights differ"
+        for k in x0:
+            assert np.array_equal(np.ascontiguousarray(x[k]).view(np.uint8),
+                                  np.ascontiguousarray(x0[k]).view(np.uint8)), k
+    if plastic:
+        with nsx.SimulationEngine.from_spec(spec) as eng:
+            w_init = eng.weights()
+        assert np.any(w0 != w_init)  # STDP changed weights

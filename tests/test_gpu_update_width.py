@@ -1,8 +1,9 @@
 """The four-neurons-per-thread update (nsx_set_update_width 4, the default from 2^21 neurons)
 against the oracle's FP32 mirror, bit for bit, at sizes that exercise its edges: neuron counts
 that are not multiples of 4 (a partial last thread, element loads) or odd (unaligned int64 ring
-pairs), pair and triplet STDP, graph windows; and a batched population of networks whose sizes
-leave padding inside segments, against the one-per-thread update."""
+pairs), pair and triplet STDP, LIF with pair STDP at both widths, graph windows; and a batched
+population of networks whose sizes leave padding inside segments, against the one-per-thread
+update."""
 import numpy as np
 import pytest
 
@@ -15,10 +16,10 @@ def _bits(a):
     return np.ascontiguousarray(a).view(np.uint8)
 
 
-def _vs_mirror(nsx, orc, spec, steps, fields, graph):
-    print(f"case width4 {spec.name} n={spec.n} steps={steps} graph={graph}", flush=True)
+def _vs_mirror(nsx, orc, spec, steps, fields, graph, width=4):
+    print(f"case width{width} {spec.name} n={spec.n} steps={steps} graph={graph}", flush=True)
     with nsx.SimulationEngine.from_spec(spec) as eng:
-        eng.set_update_width(4)
+        eng.set_update_width(width)
         eng.set_graph_steps(graph)
         st = eng.step(steps)
         g_steps, g_ids = eng.raster()
@@ -50,6 +51,22 @@ def test_lif_width4_matches_mirror(nsx, orc):
     st = _vs_mirror(nsx, orc, spec, 1500, ["v", "syn0", "syn1", "refractory", "spike_count"],
                     50)
     assert st.spikes > 500
+
+
+@pytest.mark.parametrize("width", [1, 4])
+def test_lif_pair_stdp_matches_mirror(nsx, orc, width):
+    """LIF with pair STDP (the 10^8-neuron workload's model and rule) at both update widths."""
+    spec = configs.lif_network(n=5001, seed=3, drive_scale=60.0, plasticity=True,
+                               record_raster=True, raster_capacity=1 << 22)
+    fields = ["v", "syn0", "syn1", "refractory", "spike_count", "pre_trace", "post_trace",
+              "post_step"]
+    st = _vs_mirror(nsx, orc, spec, 1500, fields, 50, width)
+    assert st.spikes > 500
+    with nsx.SimulationEngine.from_spec(spec) as eng:
+        w0 = eng.weights()
+    mir = orc.MirrorNet.from_spec(spec)
+    mir.run(1500)
+    assert np.any(mir.weights() != w0)  # the rule changed weights
 
 
 def test_batch_width4_equals_width1(nsx):

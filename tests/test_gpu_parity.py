@@ -175,6 +175,22 @@ def test_hh_substeps_bitexact(nsx, orc):
     _assert_same(st, g, m)
 
 
+@pytest.mark.parametrize("rule", ["pair", "triplet"])
+def test_hh_stdp_bitexact(nsx, orc, rule):
+    """HH (AMPA/GABA-A/NMDA, OU drive) with pair or triplet STDP on its E->E synapses."""
+    spec = nsx.configs.hh_network(n=2500, seed=12, dt_ms=0.05, hh_substeps=5, w_exc=(0.0, 0.02),
+                                  w_inh=(-0.05, 0.0), record_raster=True,
+                                  raster_capacity=1 << 22)
+    spec.cfg.plasticity = nsx.abi.PLAST_PAIR if rule == "pair" else nsx.abi.PLAST_TRIPLET
+    spec.cfg.w_min, spec.cfg.w_max = 0.0, 0.05
+    fields = ["v", "m", "h", "n", "syn0", "syn1", "syn2", "drive", "pre_trace", "post_trace",
+              "post_step"]
+    st, g, m = _run_pair(nsx, orc, spec, 1500, fields, graph_steps=50)
+    assert st.spikes > 500
+    _assert_same(st, g, m)
+    assert not np.array_equal(_bits(g[3]), _bits(orc.MirrorNet.from_spec(spec).weights()))
+
+
 def test_hh_reference_reset_bitexact(nsx, orc):
     spec = nsx.configs.hh_network(n=1500, seed=6, record_raster=True, raster_capacity=1 << 22)
     spec.cfg.hh_spike_mode = nsx.abi.HH_SPIKE_REFERENCE_RESET

@@ -1009,21 +1009,23 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_hist(StepParams p, DevPtrs 
   for (uint32_t k = threadIdx.x; k < nb; k += kBinThreads) row[k] = h[k];
 }
 
-// 16 buckets per workgroup of 256 threads: thread (column c, row group rg) holds its column's
-// counts of groups [rg * per, (rg + 1) * per) in registers (per <= 64, groups <= 1024), the 16
-// row-group sums of a column are scanned in LDS, and the exclusive offsets are written back in
-// place; the column totals go to bin_tot.  Each load instruction reads 64-byte row pieces.
-constexpr uint32_t kScanThreads = 256;
+// kScanCols buckets per workgroup of 256 threads: thread (column c, row group rg) holds its
+// column's counts of groups [rg * per, (rg + 1) * per) in registers (per <= kScanPer, groups <=
+// 1024), the row-group sums of a column are scanned in LDS, and the exclusive offsets are written
+// back in place; the column totals go to bin_tot.  Each load instruction reads row pieces of
+// kScanCols counts.
+constexpr uint32_t kScanThreads = 256, kScanCols = 16;
+constexpr uint32_t kScanRows = kScanThreads / kScanCols, kScanPer = 1024 / kScanRows;
 __global__ __launch_bounds__(kScanThreads) void k_bin_scan(StepParams p, DevPtrs s) {
-  __shared__ uint32_t part[16][16];
+  __shared__ uint32_t part[kScanRows][kScanCols];
   const uint32_t nb = p.bin_nbins, G = p.bin_groups;
-  const uint32_t c = threadIdx.x & 15u, rg = threadIdx.x >> 4;
-  const uint32_t col = blockIdx.x * 16u + c;
-  const uint32_t per = (G + 15u) / 16u;
+  const uint32_t c = threadIdx.x % kScanCols, rg = threadIdx.x / kScanCols;
+  const uint32_t col = blockIdx.x * kScanCols + c;
+  const uint32_t per = (G + kScanRows - 1) / kScanRows;
   const uint32_t r0 = rg * per;
-  uint32_t v[64], sum = 0;
+  uint32_t v[kScanPer], sum = 0;
 #pragma unroll
-  for (uint32_t k = 0; k < 64; ++k) {
+  for (uint32_t k = 0; k < kScanPer; ++k) {
     const uint32_t r = r0 + k;
     v[k] = (k < per && r < G && col < nb) ? s.bin_hist[(size_t)r * nb + col] : 0u;
     sum += v[k];
@@ -1032,13 +1034,13 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan(StepParams p, DevPtrs
   __syncthreads();
   uint32_t run = 0, tot = 0;
 #pragma unroll
-  for (uint32_t g = 0; g < 16; ++g) {
+  for (uint32_t g = 0; g < kScanRows; ++g) {
     const uint32_t x = part[g][c];
     run += g < rg ? x : 0u;
     tot += x;
   }
 #pragma unroll
-  for (uint32_t k = 0; k < 64; ++k) {
+  for (uint32_t k = 0; k < kScanPer; ++k) {
     const uint32_t r = r0 + k;
     if (k < per && r < G && col < nb) s.bin_hist[(size_t)r * nb + col] = run;
     run += v[k];
@@ -1334,7 +1336,8 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
       hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_TRIPLET>, gb, bb, 0, ctx->stream, p, ctx->d);
     else
       hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_NONE>, gb, bb, 0, ctx->stream, p, ctx->d);
-    hipLaunchKernelGGL(k_bin_scan, dim3((p.bin_nbins + 15) / 16), dim3(kScanThreads), 0,
+    hipLaunchKernelGGL(k_bin_scan, dim3((p.bin_nbins + kScanCols - 1) / kScanCols),
+                       dim3(kScanThreads), 0,
                        ctx->stream, p, ctx->d);
     if (evs) (void)hipEventRecord(evs[2], ctx->stream);
     if (pl == NSX_PLAST_PAIR)

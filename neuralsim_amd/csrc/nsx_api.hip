@@ -775,7 +775,11 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
     return set_error(ctx, NSX_E_STATE, "surrogate parameters not set (train or load them)");
   if (ctx->bin_dirty) {  // binned delivery sized for the network now loaded
     nsx_status sb = bin_configure(ctx, ctx->delivery, ctx->bin_cap_req, ctx->bin_te_req);
-    if (sb) return sb;
+    if (sb) {  // a network binned delivery cannot take: the engine stays atomic
+      ctx->delivery = NSX_DELIVERY_ATOMIC;
+      ctx->bin_dirty = false;
+      return sb;
+    }
   }
   if (ctx->p.stats_on && n_steps > ctx->p.stat_cap) {
     // the per-step count ring holds stat_cap steps: run in chunks, folding after each

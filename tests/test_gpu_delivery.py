@@ -8,7 +8,8 @@ delivery and to the oracle's FP32 mirror:
 * against atomic delivery on networks of several tiles, for every tile size;
 * with a bucket buffer too small for most steps (those steps fall back to atomics);
 * across read_state (pending buckets join the ring), upload into an atomic engine, set_step,
-  switching modes mid-run and replacing the network.
+  switching modes mid-run and replacing the network;
+* networks beyond its limits are refused, and leave the engine atomic.
 """
 import numpy as np
 import pytest
@@ -172,3 +173,22 @@ def test_binned_delivery_rejects_unsupported_shapes(nsx):
             eng.set_delivery(BINNED, 0, 1000)  # not a supported tile size
         assert eng.delivery == ATOMIC
         eng.step(10)
+
+
+def test_binned_delivery_limits(nsx):
+    """A network beyond binned delivery's limits: refused when loaded, and when chosen before
+    the network (sized at the first step) that step fails and the engine continues atomic."""
+    spec = configs.hh_network(n=200_000, seed=3, k_in=5)  # D - 1 = 200, 98 tiles of 2048
+    print("case delivery limits hh_200000", flush=True)
+    with nsx.SimulationEngine.from_spec(spec) as eng:
+        with pytest.raises(Exception):
+            eng.set_delivery(BINNED, 0, 4096)
+        assert eng.delivery == ATOMIC
+    with nsx.SimulationEngine(spec.cfg) as eng:
+        eng.set_delivery(BINNED, 0, 4096)  # no network yet: accepted, sized later
+        eng.generate_fixed_indegree(*spec.generator_args())
+        with pytest.raises(Exception):
+            eng.step(1)
+        assert eng.delivery == ATOMIC
+        st = eng.step(20)
+        assert st.steps_done == 20

@@ -1,0 +1,60 @@
+"""Misuse of the C ABI is a status, not a fault, and leaves the engine as it was: every
+rejected call below is followed by a run that must equal a clean engine's run bit for bit."""
+import numpy as np
+import pytest
+
+from neuralsim_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _status(nsx, fn, *a):
+    with pytest.raises(nsx.NsxError) as e:
+        fn(*a)
+    return e.value.status_name
+
+
+def _run(eng, steps=40):
+    st = eng.step(steps)
+    return st.total_spikes, st.total_events, eng.raster(), eng.read_state(["v", "u"])
+
+
+def test_rejected_calls_change_nothing(nsx):
+    spec = configs.izh_stdp_network(n=3000, seed=4, record_raster=True, raster_capacity=1 << 20)
+    print("case api errors izh_stdp_3000", flush=True)
+    with nsx.SimulationEngine.from_spec(spec) as clean:
+        want = _run(clean)
+    with nsx.SimulationEngine(spec.cfg) as eng:
+        assert _status(nsx, eng.step, 1) == "NSX_E_STATE"  # no network yet
+        assert _status(nsx, eng.upload_weights, np.zeros(10, np.float32)) == "NSX_E_STATE"
+        eng.generate_fixed_indegree(*spec.generator_args())
+        w = eng.weights()
+        assert _status(nsx, eng.upload_weights, w[:-1]) == "NSX_E_INVALID"
+        assert _status(nsx, eng.set_update_width, 3) == "NSX_E_INVALID"
+        assert _status(nsx, eng.set_graph_steps, 5000) == "NSX_E_INVALID"
+        assert _status(nsx, eng.set_step, 1 << 30) == "NSX_E_INVALID"
+        assert _status(nsx, eng.set_delivery, 7) == "NSX_E_INVALID"
+        assert eng.delivery == abi.DELIVERY_ATOMIC
+        eng.stimulate(spec.n, 5.0)  # out of range: ignored, as SimulationEngine::stimulate
+        got = _run(eng)
+    assert got[0] == want[0] and got[1] == want[1]
+    assert np.array_equal(got[2][0], want[2][0]) and np.array_equal(got[2][1], want[2][1])
+    for k in want[3]:
+        assert np.array_equal(got[3][k].view(np.uint32), want[3][k].view(np.uint32)), k
+
+
+def test_batch_and_surrogate_preconditions(nsx):
+    base = configs.population_base()
+    nets = configs.population_nets(3, seed=5, n_range=(300, 600))
+    plastic = base.copy()
+    plastic.plasticity = abi.PLAST_PAIR
+    assert _status(nsx, nsx.BatchedPopulation, plastic, nets) == "NSX_E_INVALID"
+    sur = base.copy()
+    sur.model = abi.MODEL_HH_SURROGATE
+    assert _status(nsx, nsx.BatchedPopulation, sur, nets) == "NSX_E_INVALID"
+    bad = configs.population_nets(1, seed=5, n_range=(300, 600))
+    bad[0].n_exc = bad[0].n_neurons + 1
+    assert _status(nsx, nsx.BatchedPopulation, base, bad) == "NSX_E_INVALID"
+    spec = configs.hh_surrogate_network(n=500, k_in=10)
+    with nsx.SimulationEngine.from_spec(spec) as eng:
+        assert _status(nsx, eng.step, 1) == "NSX_E_STATE"  # no surrogate parameters yet

@@ -90,14 +90,30 @@ def _timed(eng, steps, graph):
     return time.perf_counter() - t0, st
 
 
-def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50):
+HH_LEGS = ("surrogate", "plain_dt0.01", "plain_dt0.05_x5", "accuracy")
+
+
+def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS):
     """HH network of 200k neurons: the learned surrogate (dt 0.05 ms) against the plain
     integrator at dt 0.01 ms and against plain five-sub-step fusion at dt 0.05 ms; per-kernel
     MFMA rate of the surrogate update; spike displacement of the surrogate against the plain
-    FP32 integrator on an uncoupled population under the same drive."""
+    FP32 integrator on an uncoupled population under the same drive.  `legs` selects a subset
+    (one engine per process keeps a kernel trace to that leg)."""
     from neuralsim_amd import configs, surrogate
     from neuralsim_amd.metrics import spike_displacement
     res = {}
+    if "surrogate" not in legs:
+        for name, dt, sub in [("plain_dt0.01", 0.01, 1), ("plain_dt0.05_x5", 0.05, 5)]:
+            if name in legs:
+                with ns.SimulationEngine.from_spec(configs.hh_network(n=n, dt_ms=dt,
+                                                                      hh_substeps=sub)) as eng:
+                    steps = int(round(sim_ms / dt))
+                    wall, st = _timed(eng, steps, graph)
+                    res[name] = {"dt_ms": dt, "substeps": sub,
+                                 "ms_per_sim_s": wall / sim_ms * 1e6,
+                                 "us_per_step": wall / steps * 1e6,
+                                 "rate_hz": st.spikes / n / (sim_ms / 1e3)}
+        return res
     spec = configs.hh_surrogate_network(n=n)
     with ns.SimulationEngine.from_spec(spec) as eng:
         fit = surrogate.fit(eng)
@@ -120,6 +136,8 @@ def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50):
         res["surrogate"]["update_kernel_us"] = us
         res["surrogate"]["mlp_tflops"] = flop / (us * 1e-6) / 1e12
         res["surrogate"]["mlp_frac_of_bf16_peak"] = flop / (us * 1e-6) / 2.5e15
+    if tuple(legs) == ("surrogate",):
+        return res
     for name, dt, sub in [("plain_dt0.01", 0.01, 1), ("plain_dt0.05_x5", 0.05, 5)]:
         with ns.SimulationEngine.from_spec(configs.hh_network(n=n, dt_ms=dt,
                                                               hh_substeps=sub)) as eng:
@@ -240,6 +258,8 @@ def main():
     ap.add_argument("--no-hh", action="store_true", help="skip the HH surrogate comparison")
     ap.add_argument("--no-pop", action="store_true", help="skip the batched population")
     ap.add_argument("--no-large", action="store_true", help="skip the 10^8-neuron network")
+    ap.add_argument("--hh-leg", choices=HH_LEGS[:3],
+                    help="run only this HH leg (one engine) and print its figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU (asked for --gpus {args.gpus}); runs on several "
@@ -247,6 +267,10 @@ def main():
         sys.exit(2)
 
     import neuralsim_amd as ns
+    if args.hh_leg:
+        print(json.dumps({"hh_leg": args.hh_leg,
+                          **bench_hh(ns, legs=(args.hh_leg,))}), flush=True)
+        return
 
     spec = ns.configs.izh_stdp_network(n=args.neurons, steps=args.steps)
     t0 = time.perf_counter()

@@ -230,8 +230,8 @@ int ref_izh_step(double* v, double* u, double a, double b, double c, double d, d
 /* izhikevich.rs:20-52 with the V integration split in `sub` equal sub-steps of dt/sub (the
  * benchmark network uses two 0.5 ms V sub-steps per 1 ms step); u advances once with the
  * step-start v as the reference's du does.  sub = 1 is ref_izh_step exactly. */
-static int ref_izh_step_sub(double* v, double* u, double a, double b, double c, double d,
-                            double dt, int sub, double input_current) {
+int ref_izh_step_sub(double* v, double* u, double a, double b, double c, double d, double dt,
+                     int sub, double input_current) {
   if (*v >= 30.0) { *v = c; *u += d; return 1; }
   double v0 = *v, uu = *u, vv = *v, h = dt / sub;
   for (int s = 0; s < sub; ++s) vv += (0.04 * vv * vv + 5.0 * vv + 140.0 - uu + input_current) * h;
@@ -265,9 +265,9 @@ double ref_hh_beta_n(double v) { return 0.125 * exp(-(v + 65.0) / 80.0); }
  *   potential;
  * documented = 1: the membrane equation of the file's doc comment (hodgkin_huxley.rs:6-9),
  *   C dV/dt = g_Na m^3 h (E_Na - V) + g_K n^4 (E_K - V) + g_L (E_L - V) + I. */
-static double ref_hh_integrate(double* v, double* m, double* h, double* n, double g_na,
-                               double g_k, double g_l, double e_na, double e_k, double e_l,
-                               double c_m, double dt, double input_current, int documented) {
+double ref_hh_integrate(double* v, double* m, double* h, double* n, double g_na, double g_k,
+                        double g_l, double e_na, double e_k, double e_l, double c_m, double dt,
+                        double input_current, int documented) {
   double vv = *v;
   double am = ref_hh_alpha_m(vv), bm = ref_hh_beta_m(vv);
   double ah = ref_hh_alpha_h(vv), bh = ref_hh_beta_h(vv);
@@ -426,6 +426,16 @@ void ref_stdp_literal(uint64_t m, const uint32_t* src, const uint32_t* tgt, doub
   free(pre); free(post); free(lpre); free(lpost); free(fired);
 }
 
+/* Ornstein-Uhlenbeck input current over one step of dt, exact in distribution
+ * (dX = (mean - X)/tau dt + sigma sqrt(2/tau) dW): with e = exp(-dt/tau),
+ * X' = X e + mean (1 - e) + sigma sqrt(1 - e^2) z, z ~ N(0, 1).  The stationary law is
+ * N(mean, sigma^2) with lag-k autocorrelation e^k.  (The 200k-neuron HH workload: mean 6,
+ * sigma 3, tau 5 ms; the reference has no OU source -- engine.rs:273-287 injects uniform noise.) */
+double ref_ou_update(double x, double mean, double sigma, double tau, double dt, double z) {
+  double e = exp(-dt / tau);
+  return x * e + mean * (1.0 - e) + sigma * sqrt(1.0 - e * e) * z;
+}
+
 /* ---- FP64 network run ---------------------------------------------------------------- */
 struct ref_net {
   nsx_config cfg;
@@ -580,9 +590,8 @@ uint64_t ref_net_run(ref_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* 
         double iext = 0.0;
         if (cfg->drive == NSX_DRIVE_OU) {
           iext = r->drive[i];
-          double e = exp(-dt / cfg->drive_tau_ms);
-          r->drive[i] = r->drive[i] * e + cfg->drive_mean * (1.0 - e) +
-                        cfg->drive_sigma * sqrt(1.0 - e * e) * z;
+          r->drive[i] = ref_ou_update(r->drive[i], cfg->drive_mean, cfg->drive_sigma,
+                                      cfg->drive_tau_ms, dt, z);
         } else if (cfg->drive == NSX_DRIVE_GAUSSIAN) {
           iext = cfg->drive_mean + cfg->drive_sigma * z;
         }
@@ -648,6 +657,9 @@ void ref_net_state(ref_net* r, double* v, double* u, double* m, double* h, doubl
 }
 
 void ref_net_weights(ref_net* r, double* w) { memcpy(w, r->w, r->m * 8); }
+
+/* The OU drive current of every neuron (HH networks with an OU drive). */
+void ref_net_drive(ref_net* r, double* x) { memcpy(x, r->drive, r->n * 8); }
 
 /* ===================================================================================== */
 /* FP32 mirror of the device engine                                                       */

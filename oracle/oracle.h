@@ -50,6 +50,16 @@ double ref_hh_beta_n(double v);
 int ref_hh_step(double* v, double* m, double* h, double* n, double g_na, double g_k,
                 double g_l, double e_na, double e_k, double e_l, double c_m, double dt,
                 double input_current);
+/* izhikevich.rs:20-52 with the V integration in `sub` sub-steps of dt/sub, u from the old v. */
+int ref_izh_step_sub(double* v, double* u, double a, double b, double c, double d, double dt,
+                     int sub, double input_current);
+/* hodgkin_huxley.rs:46-86 gate and membrane update; documented = 1: the doc-comment equation
+ * (hodgkin_huxley.rs:6-9), 0: the code's signs.  Returns the step-start V. */
+double ref_hh_integrate(double* v, double* m, double* h, double* n, double g_na, double g_k,
+                        double g_l, double e_na, double e_k, double e_l, double c_m, double dt,
+                        double input_current, int documented);
+/* One exact OU step of the drive current. */
+double ref_ou_update(double x, double mean, double sigma, double tau, double dt, double z);
 void ref_finalize(uint64_t n, uint64_t m, const uint32_t* src, const uint32_t* dst,
                   uint64_t* row_ptr, uint32_t* col, uint64_t* perm);
 double ref_mg_block(double mg_block_strength, double v);
@@ -82,6 +92,7 @@ void ref_net_destroy(ref_net* r);
 uint64_t ref_net_run(ref_net* r, uint32_t n_steps, uint32_t* sp_step, uint32_t* sp_id,
                      uint64_t cap);
 void ref_net_state(ref_net* r, double* v, double* u, double* m, double* h, double* n);
+void ref_net_drive(ref_net* r, double* x);
 void ref_net_weights(ref_net* r, double* w);
 
 /* ---- FP32 mirror of the device engine ------------------------------------------------ */

@@ -35,6 +35,9 @@ PROTOS = {
     "ref_hh_alpha_h": (_f64, [_f64]), "ref_hh_beta_h": (_f64, [_f64]),
     "ref_hh_alpha_n": (_f64, [_f64]), "ref_hh_beta_n": (_f64, [_f64]),
     "ref_hh_step": (_int, [_dp, _dp, _dp, _dp] + [_f64] * 9),
+    "ref_izh_step_sub": (_int, [_dp, _dp, _f64, _f64, _f64, _f64, _f64, _int, _f64]),
+    "ref_hh_integrate": (_f64, [_dp, _dp, _dp, _dp] + [_f64] * 9 + [_int]),
+    "ref_ou_update": (_f64, [_f64] * 6),
     "ref_finalize": (None, [_u64, _u64, _P, _P, _P, _P, _P]),
     "ref_mg_block": (_f64, [_f64, _f64]),
     "ref_conductance": (_f64, [_f64, _f64, _f64, _f64]),
@@ -50,6 +53,7 @@ PROTOS = {
     "ref_net_run": (_u64, [_VP, _u32, _P, _P, _u64]),
     "ref_net_state": (None, [_VP, _P, _P, _P, _P, _P]),
     "ref_net_weights": (None, [_VP, _P]),
+    "ref_net_drive": (None, [_VP, _P]),
     "mir_create": (_VP, [C.POINTER(abi.NsxConfig)]),
     "mir_destroy": (None, [_VP]),
     "mir_build_csr": (_int, [_VP, _P, _P, _P, _P, _u64]),
@@ -261,3 +265,8 @@ class RefNet:
         w = np.zeros(self.m)
         lib().ref_net_weights(self._h, _p(w))
         return w
+
+    def drive(self):
+        x = np.zeros(int(self.cfg.n_neurons))
+        lib().ref_net_drive(self._h, _p(x))
+        return x

@@ -1,0 +1,242 @@
+"""Pins of the oracle functions that no reference test covers, against what the mathematics and
+the published model fix (CPU only):
+
+* standard-mode Hodgkin-Huxley -- the membrane equation of the reference's doc comment
+  (hodgkin_huxley.rs:6-9), used by the 200k-neuron HH workload, the surrogate's training data
+  and its FP64 judge: resting potential, gate steady states from the rate closed forms, the
+  rheobase of repetitive firing, the firing rate and spike shape at I = 10 uA/cm^2 (Hodgkin &
+  Huxley 1952; the squid-axon constants of hodgkin_huxley.rs:98-114);
+* the Ornstein-Uhlenbeck drive: stationary mean and variance and the lag-k autocorrelation
+  e^(-k dt / tau) of the exact update, in the FP64 network run and in the FP32 mirror;
+* the Izhikevich step with two V sub-steps (izhikevich.rs:20-52 split as the 1M workload runs
+  it): a hand-computed case that separates the old-v and new-v forms of du.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from neuralsim_amd import abi, configs
+
+HH = (120.0, 36.0, 0.3, 50.0, -77.0, -54.387, 1.0)  # g_Na g_K g_L E_Na E_K E_L C_m
+
+
+def _hh_trace(L, I, T_ms, dt=0.01, documented=1):
+    v, m, h, n = C.c_double(-65.0), C.c_double(0.05), C.c_double(0.6), C.c_double(0.32)
+    out = np.empty(int(round(T_ms / dt)))
+    for k in range(out.size):
+        L.ref_hh_integrate(C.byref(v), C.byref(m), C.byref(h), C.byref(n), *HH, dt, I,
+                           documented)
+        out[k] = v.value
+    return out, (m.value, h.value, n.value)
+
+
+def _upward_crossings(vs, level=0.0):
+    return np.nonzero((vs[:-1] < level) & (vs[1:] >= level))[0] + 1
+
+
+def _steady_state(L, v):
+    am, bm = L.ref_hh_alpha_m(v), L.ref_hh_beta_m(v)
+    ah, bh = L.ref_hh_alpha_h(v), L.ref_hh_beta_h(v)
+    an, bn = L.ref_hh_alpha_n(v), L.ref_hh_beta_n(v)
+    return am / (am + bm), ah / (ah + bh), an / (an + bn)
+
+
+def test_hh_standard_rest(orc):
+    """I = 0: the squid-axon constants put rest at -65 mV (E_L = -54.387 balances the Na and K
+    currents there); the gates settle at their steady states x_inf = a / (a + b)."""
+    L = orc.lib()
+    vs, gates = _hh_trace(L, 0.0, 300.0)
+    assert abs(vs[-1] + 65.0) < 0.05
+    assert np.ptp(vs[-5000:]) < 1e-3  # at rest, not oscillating
+    for g, g_inf in zip(gates, _steady_state(L, vs[-1])):
+        assert abs(g - g_inf) < 2e-4
+    # literature values of the steady states at -65 mV
+    m_inf, h_inf, n_inf = _steady_state(L, -65.0)
+    assert abs(m_inf - 0.0529) < 2e-4 and abs(h_inf - 0.5961) < 2e-4 and abs(n_inf - 0.3177) < 2e-4
+    # the code's signs (documented = 0) repel V from every reversal potential: no rest at -65
+    vs_lit, _ = _hh_trace(L, 0.0, 20.0, documented=0)
+    assert not (abs(vs_lit[-1] + 65.0) < 1.0)
+
+
+def test_hh_standard_rheobase_and_repetitive_firing(orc):
+    """Below the rheobase of repetitive firing (about 6.2 uA/cm^2) a constant current gives at
+    most the onset spike; at I = 10 the model fires tonically at about 68 Hz, the first spike
+    peaking near +40 mV and the afterhyperpolarisation near -75 mV."""
+    L = orc.lib()
+    vs5, _ = _hh_trace(L, 5.0, 300.0)
+    ups5 = _upward_crossings(vs5)
+    assert len(ups5) <= 1 and np.all(ups5 * 0.01 < 50.0)
+    vs10, _ = _hh_trace(L, 10.0, 300.0)
+    ups = _upward_crossings(vs10)
+    late = ups[ups * 0.01 > 100.0]
+    rate = (len(late) - 1) / ((late[-1] - late[0]) * 0.01 / 1000.0)
+    assert 65.0 <= rate <= 72.0
+    peaks = [vs10[a:b].max() for a, b in zip(ups[:-1], ups[1:])]
+    assert 35.0 <= peaks[0] <= 45.0
+    assert all(25.0 <= p <= 40.0 for p in peaks[3:])
+    ahp = vs10[ups[-2]:ups[-1]].min()
+    assert -78.0 <= ahp <= -72.0
+
+
+def test_hh_standard_fp32_mirror_physics(orc):
+    """The FP32 mirror's standard-mode step (orc_hh_plain) shows the same rest and tonic rate,
+    independently of the FP64 restatement."""
+    L = orc.lib()
+    cfg = configs.hh_network(n=1).cfg
+    cfg.dt_ms = 0.01
+
+    def trace(I, T):
+        cells = [np.array([x], np.float32) for x in (-65.0, 0.05, 0.6, 0.32)]
+        out = np.empty(int(round(T / 0.01)), np.float32)
+        for k in range(out.size):
+            L.orc_hh_plain(C.byref(cfg), *[C.c_void_p(c.ctypes.data) for c in cells], I, 1)
+            out[k] = cells[0][0]
+        return out
+
+    rest = trace(0.0, 300.0)
+    assert abs(float(rest[-1]) + 65.0) < 0.05
+    vs = trace(10.0, 300.0).astype(np.float64)
+    ups = _upward_crossings(vs)
+    late = ups[ups * 0.01 > 100.0]
+    rate = (len(late) - 1) / ((late[-1] - late[0]) * 0.01 / 1000.0)
+    assert 65.0 <= rate <= 72.0
+
+
+def _ou_stats(x, lags):
+    """x[t, i]: stationary samples of independent neurons; mean, sd and lag autocorrelations."""
+    mu = x.mean()
+    d = x - mu
+    var = (d * d).mean()
+    rho = [float((d[:-k] * d[k:]).mean() / var) for k in lags]
+    return float(mu), float(math.sqrt(var)), rho
+
+
+def _ou_spec(n):
+    # uncoupled HH neurons (zero weights) whose OU drive is the only thing read
+    spec = configs.hh_network(n=n, k_in=1, w_exc=(0.0, 0.0), w_inh=(0.0, 0.0), dt_ms=0.05,
+                              hh_substeps=1)
+    return spec
+
+
+def test_ou_drive_fp64_stationary_statistics(orc):
+    """The FP64 network's OU drive (mean 6, sigma 3, tau 5 ms, dt 0.05 ms): stationary mean and
+    sd within sampling error, autocorrelation at lags 1, 20, 100 equal to e^(-k dt / tau)."""
+    n, dt, tau = 2000, 0.05, 5.0
+    spec = _ou_spec(n)
+    rp = np.zeros(n + 1, np.uint64)
+    ref = orc.RefNet(spec.cfg, rp, np.zeros(0, np.uint32), np.zeros(0, np.float32),
+                     np.zeros(0, np.uint8))
+    ref.run(500)  # 25 ms = 5 tau: the variance has relaxed from 0 to within e^-10 of sigma^2
+    xs = []
+    for _ in range(1200):
+        ref.run(1)
+        xs.append(ref.drive())
+    ref.close()
+    x = np.array(xs)
+    lags = (1, 20, 100)
+    mu, sd, rho = _ou_stats(x, lags)
+    # effective samples: n neurons x T / (2 tau) correlation times
+    n_eff = n * (1200 * dt) / (2 * tau)
+    assert abs(mu - 6.0) < 5 * 3.0 / math.sqrt(n_eff)
+    assert abs(sd - 3.0) < 5 * 3.0 / math.sqrt(2 * n_eff)
+    for k, r in zip(lags, rho):
+        assert abs(r - math.exp(-k * dt / tau)) < 0.03, (k, r)
+
+
+def test_ou_drive_fp32_mirror_statistics(orc):
+    """The FP32 mirror's OU drive (the device's arithmetic) has the same stationary law."""
+    n, dt, tau = 2000, 0.05, 5.0
+    spec = _ou_spec(n)
+    mir = orc.MirrorNet(spec.cfg)
+    mir.build_csr(np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.float32),
+                  np.zeros(0, np.uint8))
+    mir.run(500)
+    xs = []
+    for _ in range(1200):
+        mir.run(1)
+        xs.append(mir.read_state(["drive"])["drive"].astype(np.float64))
+    mir.close()
+    x = np.array(xs)
+    lags = (1, 20, 100)
+    mu, sd, rho = _ou_stats(x, lags)
+    n_eff = n * (1200 * dt) / (2 * tau)
+    assert abs(mu - 6.0) < 5 * 3.0 / math.sqrt(n_eff)
+    assert abs(sd - 3.0) < 5 * 3.0 / math.sqrt(2 * n_eff)
+    for k, r in zip(lags, rho):
+        assert abs(r - math.exp(-k * dt / tau)) < 0.03, (k, r)
+
+
+def test_ou_update_closed_form_special_cases(orc):
+    """sigma = 0: deterministic exponential relaxation to the mean; dt >> tau: the draw alone
+    (X' = mean + sigma z); dt -> 0: no change."""
+    L = orc.lib()
+    assert abs(L.ref_ou_update(10.0, 6.0, 0.0, 5.0, 5.0, 1.7) - (6.0 + 4.0 * math.exp(-1.0))) < 1e-12
+    assert abs(L.ref_ou_update(10.0, 6.0, 3.0, 5.0, 500.0, 0.5) - 7.5) < 1e-12
+    assert abs(L.ref_ou_update(10.0, 6.0, 3.0, 5.0, 1e-12, 0.5) - 10.0) < 1e-5
+
+
+def test_izhikevich_two_substeps_hand_computed(orc):
+    """v = -70, u = -14, I = 3, dt = 1 ms in two 0.5 ms V sub-steps:
+    dv1 = (0.04*4900 - 350 + 140 + 14 + 3) * 0.5 = 1.5          -> v = -68.5
+    dv2 = (0.04*4692.25 - 342.5 + 140 + 14 + 3) * 0.5 = 1.095    -> v = -67.405
+    du  = a (b v_old - u) dt = 0.02 (0.2*(-70) + 14) = 0          -> u = -14 (old v; the new v
+    would give +0.01038)."""
+    L = orc.lib()
+    v, u = C.c_double(-70.0), C.c_double(-14.0)
+    assert L.ref_izh_step_sub(C.byref(v), C.byref(u), 0.02, 0.2, -65.0, 8.0, 1.0, 2, 3.0) == 0
+    assert abs(v.value - (-67.405)) < 1e-12
+    assert u.value == -14.0
+    # one sub-step is the reference step exactly
+    v1, u1 = C.c_double(-70.0), C.c_double(-14.0)
+    v2, u2 = C.c_double(-70.0), C.c_double(-14.0)
+    L.ref_izh_step_sub(C.byref(v1), C.byref(u1), 0.02, 0.2, -65.0, 8.0, 1.0, 1, 3.0)
+    L.ref_izh_step(C.byref(v2), C.byref(u2), 0.02, 0.2, -65.0, 8.0, 1.0, 3.0)
+    assert v1.value == v2.value and u1.value == u2.value
+    # spike check first, with sub-steps too
+    v, u = C.c_double(30.0), C.c_double(1.0)
+    assert L.ref_izh_step_sub(C.byref(v), C.byref(u), 0.02, 0.2, -65.0, 8.0, 1.0, 2, 100.0) == 1
+    assert v.value == -65.0 and u.value == 9.0
+    # sub-steps converge to the exact V flow: with u frozen over the step, dv/dt = 0.04 v^2 +
+    # 5 v + 140 - u + I has the closed form of a Riccati equation; more sub-steps approach it
+    a_, b_ = 0.04, 5.0
+    c_ = 140.0 - (-14.0) + 3.0
+    disc = 4 * a_ * c_ - b_ * b_  # > 0 here: tan solution
+
+    def exact(v0, t):
+        s = math.sqrt(disc)
+        th = math.atan((2 * a_ * v0 + b_) / s) + s * t / 2
+        return (s * math.tan(th) - b_) / (2 * a_)
+
+    errs = []
+    for sub in (1, 2, 8, 64):
+        v, u = C.c_double(-70.0), C.c_double(-14.0)
+        L.ref_izh_step_sub(C.byref(v), C.byref(u), 0.0, 0.2, -65.0, 8.0, 1.0, sub, 3.0)
+        errs.append(abs(v.value - exact(-70.0, 1.0)))
+    assert errs[0] > errs[1] > errs[2] > errs[3]
+    # forward Euler is first order: eight times the sub-steps, about an eighth of the error
+    assert 6.0 < errs[2] / errs[3] < 10.0
+
+
+def test_izhikevich_two_substeps_mirror_network_uses_them(orc):
+    """The 1M workload's config runs two V sub-steps; the FP32 mirror's uncoupled neurons follow
+    the FP64 two-sub-step step within FP32 rounding over one step from the initial state."""
+    spec = configs.izh_stdp_network(n=64, k_in=1, plasticity=False)
+    assert spec.cfg.v_substeps == 2
+    spec.cfg.drive = abi.DRIVE_NONE
+    L = orc.lib()
+    mir = orc.MirrorNet(spec.cfg)
+    mir.build_csr(np.zeros(0, np.uint32), np.zeros(0, np.uint32), np.zeros(0, np.float32),
+                  np.zeros(0, np.uint8))
+    s0 = mir.read_state(["v", "u"])
+    state = {"v": np.full(64, -70.0, np.float32), "u": np.full(64, -14.0, np.float32)}
+    mir.write_state(state)
+    mir.stimulate(0, 3.0)
+    mir.run(1)
+    s1 = mir.read_state(["v", "u"])
+    mir.close()
+    v, u = C.c_double(-70.0), C.c_double(-14.0)
+    L.ref_izh_step_sub(C.byref(v), C.byref(u), 0.02, 0.2, -65.0, 8.0, 1.0, 2, 3.0)
+    assert abs(float(s1["v"][0]) - v.value) < 1e-4
+    assert float(s1["u"][0]) == -14.0
+    del s0

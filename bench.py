@@ -1,10 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: syn events/s and x-realtime of the 1M-neuron / 100M-synapse Izhikevich + pair-STDP
-network on one MI355X, with the propagation/update kernel roofline and a CPU baseline.
+network on one MI355X, with the dominant kernel's roofline and a CPU baseline; the other
+workloads (10k LIF, 200k HH plain vs surrogate, the 256-network population, 10^8 LIF) as
+sections of the same JSON line.
 
-A step is one 1 ms simulation step of the whole network (neuron update, propagation with delays,
-STDP).  `value` = synaptic events (sum of out-degrees of spiking rows) per wall second over the
-K timed steps, inputs (network, state) resident in HBM.  Prints one JSON line.
+A bench step is one SimulationEngine::simulate_ms(100) call (engine.rs:792-797): 100 simulation
+steps of 1 ms of the whole network (neuron update, propagation with delays, STDP), so the default
+100 steps are the workload's 10 s of simulated time.  The network's activity oscillates
+(population bursts about every 0.2 s), and a 100 ms step keeps any timed window several cycles
+long.  Before any timing the network runs a declared burn-in (`config.burnin_ms`, default 1 s
+simulated) out of its quiet initial state, then the --warmup steps.  `value` = synaptic events
+(sum of out-degrees of spiking rows) per wall second over exactly the --steps timed steps, network
+and state resident in HBM.  The same steps are then replayed from a device snapshot as direct
+launches bracketed by hipEvents, so the per-kernel times and the roofline describe the timed
+window itself.  Prints one JSON line.
 """
 import argparse
 import json
@@ -16,6 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+BF16_PEAK = 2.5e15     # MI355X dense bf16 MFMA, FLOP/s
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_izh_stdp_1M_100M.json")
 
 
 def log(*a):
@@ -39,47 +50,223 @@ def neuron_bytes(cfg, n, spikes, ring=True):
     return (per_n - (0 if ring else ring_b)) * n + per_s * spikes
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE / WRITE_SIZE runs kept in
-    profiles/ (rocprofv3 --pmc; commands in that file), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_izh_stdp_1M_100M.json")
+def synapse_bytes(events, spikes, ltp_slots, ltd_slots):
+    """Algorithmic bytes of the synapse pass: propagation 4 B col + 4 B weight + 1 B delay +
+    8 B int64 accumulate per event and 16 B of row pointers per spiking row; potentiation 16 B
+    post record + 4 B weight write per E->E out-slot of a spiking row; depression 4 B slot + 4 B
+    source + 8 B pre record + 8 B weight read+write per walked E->E in-slot."""
+    prop = 17.0 * events + 16.0 * spikes
+    return prop, prop + 20.0 * ltp_slots + 24.0 * ltd_slots
+
+
+def regime_tag(args):
+    return {"workload": "izhikevich_pair_stdp_1M_100M", "neurons": args.neurons,
+            "burnin_ms": args.burnin_ms, "step_ms": args.step_ms, "warmup": args.warmup,
+            "steps": args.steps}
+
+
+def measured_traffic(kernel, tag):
+    """HBM bytes per launch of `kernel` over the timed window from the calibrated FETCH_SIZE /
+    WRITE_SIZE runs kept in profiles/ -- only when those runs counted this exact window (same
+    workload, burn-in, warm-up and steps); otherwise None."""
     try:
-        with open(path) as f:
-            return json.load(f)["kernels"][kernel]["hbm_bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+        with open(TRAFFIC_PROFILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
         return None
+    if prof.get("regime") != tag:
+        return None
+    return prof.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(spec, gpu_events_per_step, steps):
-    """FP64 restatement of the reference's step (per-synapse STDP traces, serial O(M) synapse
-    passes as engine.rs:397-537) timed on one host core over a bounded window of the same
-    network; converted to the metric's unit with the device run's mean events per step."""
-    from oracle import oracle
+# ---- CPU baseline (the reference's CPU step on the host's cores) ------------------------------
+
+def _port_sample(cfg, csr, steps, conductance=False, state=None, budget_s=12.0):
+    """Time `steps` steps (fewer when one step exceeds the budget) of the CPU port on a CSR
+    network; returns seconds per step, steps run and the port's own spikes and events."""
+    from oracle import cpu_port
+    rp, col, w, d = csr
+    net = cpu_port.PortNet(cfg, rp, col, w, d, conductance=conductance)
+    if state is not None:
+        net.set_state(*state)
+    s1, sp, ev = net.run(1)
+    done = 1
+    k = int(max(0, min(steps - 1, (budget_s - s1) / max(s1, 1e-9))))
+    if k:
+        s2, sp2, ev2 = net.run(k)
+        s1, sp, ev, done = s1 + s2, sp + sp2, ev + ev2, done + k
+    net.close()
+    return s1 / done, done, sp, ev
+
+
+def _cpu_entry(value, unit, sample, extra=None):
+    from oracle import cpu_port
+    info = cpu_port.host_info()
+    out = {"value": value, "unit": unit, "cores": info["threads"], "kind": "port",
+           "sample": sample, "cpu_model": info["cpu_model"], "nproc": info["nproc"]}
+    if extra:
+        out.update(extra)
+    return out
+
+
+# ---- workloads -------------------------------------------------------------------------------
+
+def bench_headline(ns, args):
+    """The 1M/100M Izhikevich + pair STDP network: burn-in, warm-up, K timed bench steps (graph
+    replay), then the same steps replayed with per-kernel events."""
+    spec = ns.configs.izh_stdp_network(n=args.neurons)
+    dt = spec.cfg.dt_ms
+    per = int(round(args.step_ms / dt))  # simulation steps per bench step
     t0 = time.perf_counter()
-    src, dst, w, d = oracle.generate_list(spec)
-    rp, col, perm = oracle.finalize(spec.n, src, dst)
-    w, d = w[perm], d[perm]
-    del src, dst, perm
-    ref = oracle.RefNet(spec.cfg, rp, col, w, d)
-    del rp, col, w, d
-    setup = time.perf_counter() - t0
-    log(f"cpu baseline: network ready in {setup:.1f}s, timing {steps} steps")
+    eng = ns.SimulationEngine.from_spec(spec)
+    build_s = time.perf_counter() - t0
+    log(f"bench: {spec.name} n={spec.n} m={spec.m} built on device in {build_s:.2f}s")
+    if args.split:
+        eng.set_prop_split(args.split)
+    G = args.graph
+    eng.set_graph_steps(G)
+    eng.step(int(round(args.burnin_ms / dt)))
+    eng.step(args.warmup * per)
+    log(f"bench: burn-in {args.burnin_ms:g} ms + warmup {args.warmup} x {args.step_ms:g} ms done")
+    K = args.steps
+    S = K * per  # simulation steps timed
+    Gt = min(G, S) if G else 0
+    eng.set_graph_steps(Gt)
+    eng.snapshot_save()
+    if Gt:
+        eng.step(Gt)  # capture the window's graph outside the timed region
+        eng.snapshot_restore()
     t0 = time.perf_counter()
-    ref.run(steps, cap=1 << 24)
-    dt = time.perf_counter() - t0
-    ref.close()
-    s_per_step = dt / steps
-    return {
-        "value": gpu_events_per_step / s_per_step,
+    st = eng.step(S)
+    wall = time.perf_counter() - t0
+    log(f"bench: timed {K} x {args.step_ms:g} ms in {wall:.4f}s (device {st.wall_ms / 1e3:.4f}s)")
+    sim_s = S * dt / 1000.0
+    out = {
+        "metric": "syn events/s & x-realtime @1M/100M Izh+STDP; HH surrogate vs plain speedup",
+        "value": st.events / wall,
         "unit": "syn events/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": (f"{steps} steps (from t=0) of the same 1M/100M network in the oracle's FP64 "
-                   f"restatement of SimulationEngine::step with per-synapse STDP traces, "
-                   f"{s_per_step * 1e3:.0f} ms/step on 1 core; events/s = device mean events "
-                   f"per step / CPU seconds per step"),
-        "ms_per_step": s_per_step * 1e3,
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox-generated network and drive)",
+        "config": {
+            "workload": "izhikevich_pair_stdp_1M_100M",
+            "neurons": spec.n, "synapses": spec.m, "dt_ms": dt,
+            "v_substeps": spec.cfg.v_substeps, "delays_ms": [1, 20],
+            "bench_step": f"simulate_ms({args.step_ms:g}): {per} simulation steps of {dt:g} ms",
+            "burnin_ms": args.burnin_ms,
+            "t_start_ms": args.burnin_ms + args.warmup * per * dt,
+            "sim_ms": sim_s * 1e3, "accumulate": "int64 fixed point 2^-32",
+            "launch": ({"graph_steps": Gt, "graph_launches": S // Gt, "direct_steps": S % Gt}
+                       if Gt else {"graph_steps": 0, "graph_launches": 0, "direct_steps": S}),
+        },
+        "x_realtime": sim_s / wall,
+        "ms_per_sim_step": wall * 1e3 / S,
+        "spikes": st.spikes,
+        "events": st.events,
+        "events_per_sim_step": st.events / S,
+        "mean_rate_hz": st.spikes / spec.n / sim_s,
+        "build_s": build_s,
     }
+    if not args.no_profile:
+        # the same steps again, from the snapshot, as direct launches with per-kernel events
+        eng.snapshot_restore()
+        eng.set_graph_steps(0)
+        eng.set_profiling(True)
+        pst = eng.step(S)
+        eng.set_profiling(False)
+        assert pst.events == st.events and pst.spikes == st.spikes, "replay differs"
+        kinds = {0: "k_update", 1: "k_synapses"}
+        times = {k: eng.kernel_time(k) for k in kinds}
+        ltp_slots, ltd_slots = times[1][2], eng.kernel_time(2)[2]
+        inv = 1.0 / S
+        upd_b = neuron_bytes(spec.cfg, spec.n, st.spikes * inv)
+        prop_b, syn_b = synapse_bytes(st.events * inv, st.spikes * inv, ltp_slots * inv,
+                                      ltd_slots * inv)
+        kt = {kinds[k]: {"avg_us": (ms / max(n, 1)) * 1e3, "launches": n}
+              for k, (ms, n, _) in times.items()}
+        kt["k_update"]["alg_bytes_per_launch"] = upd_b
+        kt["k_synapses"]["alg_bytes_per_launch"] = syn_b
+        kt["k_synapses"]["propagation_bytes_per_launch"] = prop_b
+        kt["k_synapses"]["events_per_launch"] = st.events * inv
+        kt["k_synapses"]["ltp_slots_per_launch"] = ltp_slots * inv
+        kt["k_synapses"]["ltd_slots_per_launch"] = ltd_slots * inv
+        for k in kinds.values():
+            kt[k]["achieved_GBs"] = (kt[k]["alg_bytes_per_launch"]
+                                     / (max(kt[k]["avg_us"], 1e-9) * 1e-6) / 1e9)
+        # one 64-bit ring atomic per event: rate against the device's measured random-atomic
+        # throughput (profiles/atomic_ceiling.md)
+        kt["k_synapses"]["atomics_G_per_s"] = st.events * inv / (kt["k_synapses"]["avg_us"] * 1e3)
+        kt["k_synapses"]["atomic_ceiling_G_per_s"] = 23.0
+        dom = max(kinds.values(), key=lambda k: kt[k]["avg_us"])
+        out["roofline"] = {
+            "kernel": dom, "bound": "hbm", "achieved": kt[dom]["achieved_GBs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kt[dom]["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dom, regime_tag(args)),
+        }
+        out["kernels"] = kt
+        kern_ms = sum(kt[k]["avg_us"] for k in kinds.values()) / 1e3
+        out["profiled_segment"] = {"sim_steps": S, "launch": "direct, hipEvents per kernel",
+                                   "same_steps_as_timed": True,
+                                   "kernel_ms_per_sim_step": kern_ms,
+                                   "ms_per_sim_step": pst.wall_ms / S}
+    cpu_in = None
+    if not args.no_cpu:
+        eng.snapshot_restore()
+        s = eng.read_state(["v", "u"])
+        cpu_in = (eng.read_csr(), (s["v"], s["u"]), spec, st.events / S)
+    eng.close()
+    return out, cpu_in
+
+
+def cpu_headline(cpu_in, budget_s):
+    """The CPU port on the same network (weights after the burn-in) from the device's state at
+    the start of the timed window; converted to the metric's unit with the device's events per
+    simulation step over the timed window (the port's O(M) passes make its step cost nearly
+    independent of activity)."""
+    csr, state, spec, ev_per_step = cpu_in
+    log("cpu baseline: 1M/100M port")
+    s_step, done, sp, ev = _port_sample(spec.cfg, csr, 8, state=state, budget_s=budget_s)
+    return _cpu_entry(
+        ev_per_step / s_step, "syn events/s",
+        f"{done} simulation steps (1 ms) of the 1M/100M network in the reference's CPU step "
+        f"(per-synapse AoS, per-synapse STDP traces, O(M) passes; OpenMP neuron phase) from the "
+        f"device's state at the start of the timed window: {s_step * 1e3:.0f} ms per step; "
+        f"value = the device's events per step over the timed window / CPU seconds per step",
+        {"ms_per_sim_step": s_step * 1e3, "x_realtime": 1e-3 / s_step,
+         "port_events_per_step": ev / done})
+
+
+def bench_lif10k(ns, cpu=True, budget_s=8.0):
+    """The CPU-runnable 10k LIF network: 1 s simulated (10^4 steps of 0.1 ms) at the stated
+    drive (silent) and with the drive x60 (active)."""
+    from neuralsim_amd import configs
+    res = {}
+    for key, scale in (("stated_drive", 1.0), ("drive_x60", 60.0)):
+        spec = configs.lif_network(n=10_000, drive_scale=scale)
+        with ns.SimulationEngine.from_spec(spec) as eng:
+            eng.set_graph_steps(100)
+            eng.step(100)  # captures the 100-step graph
+            t0 = time.perf_counter()
+            st = eng.step(10_000 - 100)
+            wall = time.perf_counter() - t0
+            r = {"steps": 10_000 - 100, "ms_per_step": wall * 1e3 / (10_000 - 100),
+                 "x_realtime": (10_000 - 100) * 0.1e-3 / wall, "syn_events_per_s": st.events / wall,
+                 "rate_hz": st.spikes / spec.n / ((10_000 - 100) * 1e-4)}
+            if cpu:
+                csr = eng.read_csr()
+                s_step, done, sp, ev = _port_sample(spec.cfg, csr, 4000, budget_s=budget_s / 2)
+                r["cpu_baseline"] = _cpu_entry(
+                    1e-4 / s_step, "x realtime",
+                    f"{done} steps (0.1 ms) of the same network in the reference's CPU step",
+                    {"ms_per_step": s_step * 1e3, "port_events_per_step": ev / done})
+            res[key] = r
+    return res
 
 
 def _timed(eng, steps, graph):
@@ -93,7 +280,7 @@ def _timed(eng, steps, graph):
 HH_LEGS = ("surrogate", "plain_dt0.01", "plain_dt0.05_x5", "accuracy")
 
 
-def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS):
+def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS, cpu=False, budget_s=8.0):
     """HH network of 200k neurons: the learned surrogate (dt 0.05 ms) against the plain
     integrator at dt 0.01 ms and against plain five-sub-step fusion at dt 0.05 ms; per-kernel
     MFMA rate of the surrogate update; spike displacement of the surrogate against the plain
@@ -102,43 +289,33 @@ def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS):
     from neuralsim_amd import configs, surrogate
     from neuralsim_amd.metrics import spike_displacement
     res = {}
-    if "surrogate" not in legs:
-        for name, dt, sub in [("plain_dt0.01", 0.01, 1), ("plain_dt0.05_x5", 0.05, 5)]:
-            if name in legs:
-                with ns.SimulationEngine.from_spec(configs.hh_network(n=n, dt_ms=dt,
-                                                                      hh_substeps=sub)) as eng:
-                    steps = int(round(sim_ms / dt))
-                    wall, st = _timed(eng, steps, graph)
-                    res[name] = {"dt_ms": dt, "substeps": sub,
-                                 "ms_per_sim_s": wall / sim_ms * 1e6,
-                                 "us_per_step": wall / steps * 1e6,
-                                 "rate_hz": st.spikes / n / (sim_ms / 1e3)}
-        return res
-    spec = configs.hh_surrogate_network(n=n)
-    with ns.SimulationEngine.from_spec(spec) as eng:
-        fit = surrogate.fit(eng)
-        params, norm = eng.surrogate_params()
-        res["fit"] = {"samples": fit["samples"], "generate_s": fit["generate_s"],
-                      "train_s": fit["train_s"], "steps": len(fit["loss"]),
-                      "final_loss": float(fit["loss"][-200:].mean())}
-        steps = int(round(sim_ms / 0.05))
-        wall, st = _timed(eng, steps, graph)
-        res["surrogate"] = {"dt_ms": 0.05, "ms_per_sim_s": wall / sim_ms * 1e6,
-                            "us_per_step": wall / steps * 1e6,
-                            "rate_hz": st.spikes / n / (sim_ms / 1e3),
-                            "fallback_neuron_steps": eng.surrogate_fallbacks()}
-        eng.set_profiling(True)
-        eng.step(200)
-        ms, launches, _ = eng.kernel_time(0)
-        eng.set_profiling(False)
-        us = ms / max(launches, 1) * 1e3
-        flop = 2.0 * (5 * 64 + 64 * 64 + 2 * 64 * 4) * n  # W3 runs as two bf16 words
-        res["surrogate"]["update_kernel_us"] = us
-        res["surrogate"]["mlp_tflops"] = flop / (us * 1e-6) / 1e12
-        res["surrogate"]["mlp_frac_of_bf16_peak"] = flop / (us * 1e-6) / 2.5e15
-    if tuple(legs) == ("surrogate",):
-        return res
+    params = norm = None
+    if "surrogate" in legs:
+        spec = configs.hh_surrogate_network(n=n)
+        with ns.SimulationEngine.from_spec(spec) as eng:
+            fit = surrogate.fit(eng)
+            params, norm = eng.surrogate_params()
+            res["fit"] = {"samples": fit["samples"], "generate_s": fit["generate_s"],
+                          "train_s": fit["train_s"], "steps": len(fit["loss"]),
+                          "final_loss": float(fit["loss"][-200:].mean())}
+            steps = int(round(sim_ms / 0.05))
+            wall, st = _timed(eng, steps, graph)
+            res["surrogate"] = {"dt_ms": 0.05, "ms_per_sim_s": wall / sim_ms * 1e6,
+                                "us_per_step": wall / steps * 1e6,
+                                "rate_hz": st.spikes / n / (sim_ms / 1e3),
+                                "fallback_neuron_steps": eng.surrogate_fallbacks()}
+            eng.set_profiling(True)
+            eng.step(200)
+            ms, launches, _ = eng.kernel_time(0)
+            eng.set_profiling(False)
+            us = ms / max(launches, 1) * 1e3
+            flop = 2.0 * (5 * 64 + 64 * 64 + 2 * 64 * 4) * n  # W3 runs as two bf16 words
+            res["surrogate"]["update_kernel_us"] = us
+            res["surrogate"]["mlp_tflops"] = flop / (us * 1e-6) / 1e12
+            res["surrogate"]["mlp_frac_of_bf16_peak"] = flop / (us * 1e-6) / BF16_PEAK
     for name, dt, sub in [("plain_dt0.01", 0.01, 1), ("plain_dt0.05_x5", 0.05, 5)]:
+        if name not in legs:
+            continue
         with ns.SimulationEngine.from_spec(configs.hh_network(n=n, dt_ms=dt,
                                                               hh_substeps=sub)) as eng:
             steps = int(round(sim_ms / dt))
@@ -146,26 +323,40 @@ def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS):
             res[name] = {"dt_ms": dt, "substeps": sub, "ms_per_sim_s": wall / sim_ms * 1e6,
                          "us_per_step": wall / steps * 1e6,
                          "rate_hz": st.spikes / n / (sim_ms / 1e3)}
-    s_ms = res["surrogate"]["ms_per_sim_s"]
-    res["speedup_vs_plain_dt0.01"] = res["plain_dt0.01"]["ms_per_sim_s"] / s_ms
-    res["speedup_vs_plain_dt0.05_x5"] = res["plain_dt0.05_x5"]["ms_per_sim_s"] / s_ms
-    # accuracy: 2000 uncoupled neurons, 500 ms, same OU drive
-    kw = dict(k_in=1, w_exc=(0.0, 0.0), w_inh=(0.0, 0.0), record_raster=True,
-              raster_capacity=1 << 22)
-    u = configs.hh_surrogate_network(n=2000, **kw)
-    with ns.SimulationEngine.from_spec(u) as eng:
-        eng.set_surrogate_params(params, norm)
-        eng.step(10_000)
-        a = eng.raster()
-    with ns.SimulationEngine.from_spec(configs.hh_network(n=2000, dt_ms=0.05, hh_substeps=5,
-                                                          **kw)) as eng:
-        eng.step(10_000)
-        b = eng.raster()
-    res["displaced_vs_plain_fp32"] = spike_displacement(b[0], b[1], a[0], a[1], 0.05, 0.5)
+            if cpu and name == "plain_dt0.01":
+                log("cpu baseline: HH 200k/20M port")
+                csr = eng.read_csr()
+                s_step, done, sp, ev = _port_sample(configs.hh_network(n=n).cfg, csr, 40,
+                                                    conductance=True, budget_s=budget_s)
+                res["cpu_baseline"] = _cpu_entry(
+                    1e-5 / s_step, "x realtime",
+                    f"{done} steps (0.01 ms) of the 200k/20M network in the reference's CPU step in "
+                    f"conductance mode (per-synapse conductances decayed and summed every step, "
+                    f"OpenMP neuron phase); value = 0.01 ms / seconds per step",
+                    {"ms_per_step": s_step * 1e3, "ms_per_sim_s": s_step * 1e5 * 1e3})
+    if "surrogate" in res:
+        s_ms = res["surrogate"]["ms_per_sim_s"]
+        for name in ("plain_dt0.01", "plain_dt0.05_x5"):
+            if name in res:
+                res[f"speedup_vs_{name}"] = res[name]["ms_per_sim_s"] / s_ms
+    if "accuracy" in legs and params is not None:
+        # 2000 uncoupled neurons, 500 ms, same OU drive
+        kw = dict(k_in=1, w_exc=(0.0, 0.0), w_inh=(0.0, 0.0), record_raster=True,
+                  raster_capacity=1 << 22)
+        u = configs.hh_surrogate_network(n=2000, **kw)
+        with ns.SimulationEngine.from_spec(u) as eng:
+            eng.set_surrogate_params(params, norm)
+            eng.step(10_000)
+            a = eng.raster()
+        with ns.SimulationEngine.from_spec(configs.hh_network(n=2000, dt_ms=0.05, hh_substeps=5,
+                                                              **kw)) as eng:
+            eng.step(10_000)
+            b = eng.raster()
+        res["displaced_vs_plain_fp32"] = spike_displacement(b[0], b[1], a[0], a[1], 0.05, 0.5)
     return res
 
 
-def bench_population(ns, steps=2000, warm=200, tune_steps=100):
+def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget_s=12.0):
     """256 Izhikevich networks (N log-uniform in [1k, 64k], in-degree uniform in [20, 200]) as
     one batched population, 2 s at dt 1 ms: the first `warm` steps (the networks leave their
     quiet initial state), autotuning from that state (trials restore it), then the remaining
@@ -187,49 +378,84 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100):
         spikes, _ = pop.net_stats()
         deliv, tile = pop.delivery, int(pop.last_tune.tile_entries)
     n = np.array([b.n_neurons for b in nets], np.float64)
+    m = np.array([b.n_neurons * b.k_in for b in nets], np.float64)
     rate = spikes / n / (steps * base.dt_ms / 1e3)
     fit = RateHomeostasis(5.0).fitness(rate)
-    return {"networks": len(nets), "neurons": int(n.sum()),
-            "synapses": int(sum(b.n_neurons * b.k_in for b in nets)), "steps": steps,
-            "build_s": build_s,
-            "tuned": {"prop_split": split, "graph_steps": graph,
-                      "delivery": "binned" if deliv else "atomic", "tile_entries": tile,
-                      "trials_us_per_step": [[int(a), int(b), round(c, 1), int(d)]
-                                             for a, b, c, d in trials]},
-            "timed_steps": steps - warm,
-            "ms_per_step": wall * 1e3 / (steps - warm),
-            "x_realtime": (steps - warm) * base.dt_ms / 1e3 / wall,
-            "syn_events_per_s": st.events / wall, "spikes": int(spikes.sum()),
-            "rate_hz": {"min": float(rate.min()), "median": float(np.median(rate)),
-                        "max": float(rate.max())},
-            "fitness_mean": float(fit.mean()), "fitness_best": float(fit.max())}
+    res = {"networks": len(nets), "neurons": int(n.sum()), "synapses": int(m.sum()),
+           "steps": steps, "build_s": build_s,
+           "tuned": {"prop_split": split, "graph_steps": graph,
+                     "delivery": "binned" if deliv else "atomic", "tile_entries": tile,
+                     "trials_us_per_step": [[int(a), int(b), round(c, 1), int(d)]
+                                            for a, b, c, d in trials]},
+           "timed_steps": steps - warm,
+           "ms_per_step": wall * 1e3 / (steps - warm),
+           "x_realtime": (steps - warm) * base.dt_ms / 1e3 / wall,
+           "syn_events_per_s": st.events / wall, "spikes": int(spikes.sum()),
+           "rate_hz": {"min": float(rate.min()), "median": float(np.median(rate)),
+                       "max": float(rate.max())},
+           "fitness_mean": float(fit.mean()), "fitness_best": float(fit.max())}
+    if cpu:
+        res["cpu_baseline"] = cpu_population(base, nets, m, budget_s)
+    return res
+
+
+def cpu_population(base, nets, m, budget_s, stride=16, steps=20):
+    """Population evaluation on the CPU port (one engine per genome, genomes in parallel as
+    population.rs:116-124 does with rayon) over every `stride`-th network; the port's cost is
+    O(M) per step, so the whole population's step is extrapolated by synapse count, assuming
+    the genomes spread over all cores."""
+    import numpy as np
+    from neuralsim_amd import configs
+    from oracle import cpu_port, oracle
+    log("cpu baseline: population sample")
+    idx = list(range(0, len(nets), stride))
+    ports = []
+    for i in idx:
+        cfg = configs.population_single(base, nets[i])
+        spec = configs.NetSpec("pop", cfg, nets[i].k_in, (nets[i].w_exc_lo, nets[i].w_exc_hi),
+                               (nets[i].w_inh_lo, nets[i].w_inh_hi), (nets[i].d_lo, nets[i].d_hi), 1)
+        src, dst, w, d = oracle.generate_list(spec)
+        rp, col, perm = oracle.finalize(spec.n, src, dst)
+        ports.append(cpu_port.PortNet(cfg, rp, col, w[perm], d[perm]))
+    wall, secs = cpu_port.run_many(ports, steps)
+    for p in ports:
+        p.close()
+    threads = cpu_port.host_info()["threads"]
+    per_syn_step = secs.sum() / (m[idx].sum() * steps)  # one thread's seconds per synapse-step
+    pop_step = per_syn_step * m.sum() / threads
+    return _cpu_entry(
+        1e-3 / pop_step, "x realtime",
+        f"{len(idx)} of the 256 networks (every {stride}th), {steps} steps each, genomes in "
+        f"parallel on {threads} threads; population step = per-synapse cost x all synapses / "
+        f"threads = {pop_step * 1e3:.0f} ms (2 s simulated: {pop_step * 2000:.0f} s)",
+        {"ms_per_step": pop_step * 1e3, "sample_wall_s": wall})
 
 
 def bench_large(ns, n=100_000_000, steps=300, drive_scale=1.0, graph=50):
     """10^8 LIF neurons, 10^10 synapses, pair STDP on E->E: device build, then steps.  At the
     workload's drive the network is silent and the neuron sweep (HBM-bound) is the whole step;
-    drive_scale multiplies the Poisson weight to run the same network active."""
+    drive_scale multiplies the Poisson weight to run the same network active.  The kernel times
+    come from the next `steps` steps as direct launches with per-kernel events (a snapshot of
+    this network would not fit beside it in HBM)."""
     from neuralsim_amd import configs
     spec = configs.lif_large_network(n=n, drive_scale=drive_scale)
     t0 = time.perf_counter()
     with ns.SimulationEngine.from_spec(spec) as eng:
         build_s = time.perf_counter() - t0
         eng.set_graph_steps(graph)
-        eng.step(50)
+        eng.step(50 + graph)  # captures the graph
         t0 = time.perf_counter()
         st = eng.step(steps)
         wall = time.perf_counter() - t0
         eng.set_graph_steps(0)
         eng.set_profiling(True)
-        pst = eng.step(50)
+        pst = eng.step(steps)
         eng.set_profiling(False)
         ms_u, n_u, _ = eng.kernel_time(0)
         ms_s, n_s, _ = eng.kernel_time(1)
     upd_us = ms_u / max(n_u, 1) * 1e3
-    # a silent segment (no spike in it nor in the D - 1 steps before: st covers those) skips
-    # the ring loads
-    upd_b = neuron_bytes(spec.cfg, spec.n, pst.spikes / 50.0,
-                         ring=(pst.spikes + st.spikes) > 0)
+    # a silent segment (no spike in it nor in the D - 1 steps before) skips the ring loads
+    upd_b = neuron_bytes(spec.cfg, spec.n, pst.spikes / steps, ring=(pst.spikes + st.spikes) > 0)
     sim_s = steps * spec.cfg.dt_ms / 1e3
     return {"neurons": spec.n, "synapses": spec.m, "drive_scale": drive_scale,
             "build_s": build_s, "steps": steps,
@@ -238,28 +464,59 @@ def bench_large(ns, n=100_000_000, steps=300, drive_scale=1.0, graph=50):
             "k_update": {"avg_us": upd_us, "alg_bytes_per_launch": upd_b,
                          "achieved_GBs": upd_b / (upd_us * 1e-6) / 1e9,
                          "frac_of_hbm_peak": upd_b / (upd_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
-            "k_synapses_avg_us": ms_s / max(n_s, 1) * 1e3}
+            "k_synapses_avg_us": ms_s / max(n_s, 1) * 1e3,
+            "profiled_segment": {"steps": steps, "spikes": int(pst.spikes),
+                                 "kernel_ms_per_step": (ms_u + ms_s) / steps,
+                                 "ms_per_step": pst.wall_ms / steps}}
+
+
+def cpu_large(ns, budget_s):
+    """The 10^8-neuron network is infeasible on the CPU port (its AoS synapse state alone is
+    about 1 TB): the same network at 1/100 scale (10^6 LIF neurons, 10^8 synapses, pair STDP
+    E->E), timed over a step window; reported as such, not extrapolated across scale."""
+    from neuralsim_amd import configs
+    spec = configs.lif_large_network(n=1_000_000, drive_scale=60.0)
+    with ns.SimulationEngine.from_spec(spec) as eng:
+        csr = eng.read_csr()
+    log("cpu baseline: LIF 10^6 / 10^8 port (1/100 scale)")
+    s_step, done, sp, ev = _port_sample(spec.cfg, csr, 8, budget_s=budget_s)
+    return _cpu_entry(
+        1e-4 / s_step, "x realtime",
+        f"1/100 scale: {done} steps (0.1 ms) of a 10^6-neuron / 10^8-synapse LIF + pair STDP "
+        f"network (drive x60) in the reference's CPU step; not extrapolated to 10^8 neurons",
+        {"ms_per_step": s_step * 1e3, "scale": 0.01})
+
+
+LEGS = ("headline", "lif10k", "hh_surrogate", "hh_plain", "hh_plain_x5", "population",
+        "large", "large_active")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10_000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100,
+                    help="timed bench steps (each one simulate_ms(--step-ms))")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--step-ms", type=float, default=100.0,
+                    help="simulated ms per bench step of the headline network")
+    ap.add_argument("--burnin-ms", type=float, default=1000.0,
+                    help="simulated ms run before the warm-up (out of the quiet initial state)")
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--graph", type=int, default=50, help="steps per captured hipGraph (0: off)")
-    ap.add_argument("--no-profile", action="store_true", help="no per-kernel event segment")
-    ap.add_argument("--profile-steps", type=int, default=2000,
-                    help="steps of the direct-launch segment timed per kernel")
+    ap.add_argument("--no-profile", action="store_true", help="no per-kernel replay")
     ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
-    ap.add_argument("--cpu-steps", type=int, default=30,
-                    help="steps of the CPU baseline (about 0.5 s each on one core)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of CPU-port stepping per workload sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hh", action="store_true", help="skip the HH surrogate comparison")
     ap.add_argument("--no-pop", action="store_true", help="skip the batched population")
     ap.add_argument("--no-large", action="store_true", help="skip the 10^8-neuron network")
-    ap.add_argument("--hh-leg", choices=HH_LEGS[:3],
-                    help="run only this HH leg (one engine) and print its figures")
+    ap.add_argument("--no-lif10k", action="store_true", help="skip the 10k LIF network")
+    ap.add_argument("--leg", choices=LEGS,
+                    help="run only this workload (one engine per process, for kernel traces)")
+    ap.add_argument("--counters-window", action="store_true",
+                    help="headline network only: burn-in, warm-up and the timed steps as direct "
+                         "launches, nothing else (for rocprofv3 --pmc over that window)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU (asked for --gpus {args.gpus}); runs on several "
@@ -267,110 +524,51 @@ def main():
         sys.exit(2)
 
     import neuralsim_amd as ns
-    if args.hh_leg:
-        print(json.dumps({"hh_leg": args.hh_leg,
-                          **bench_hh(ns, legs=(args.hh_leg,))}), flush=True)
+    if args.counters_window:
+        spec = ns.configs.izh_stdp_network(n=args.neurons)
+        per = int(round(args.step_ms / spec.cfg.dt_ms))
+        before = int(round(args.burnin_ms / spec.cfg.dt_ms)) + args.warmup * per
+        with ns.SimulationEngine.from_spec(spec) as eng:
+            eng.set_graph_steps(0)
+            eng.step(before)
+            st = eng.step(args.steps * per)
+        print(json.dumps({"counters_window": regime_tag(args),
+                          "k_synapses_launches_before": before,
+                          "k_synapses_launches_window": args.steps * per,
+                          "events": st.events, "spikes": st.spikes}), flush=True)
+        return
+    if args.leg and args.leg != "headline":
+        res = {"leg": args.leg}
+        if args.leg == "lif10k":
+            res.update(bench_lif10k(ns, cpu=False))
+        elif args.leg.startswith("hh"):
+            leg = {"hh_surrogate": "surrogate", "hh_plain": "plain_dt0.01",
+                   "hh_plain_x5": "plain_dt0.05_x5"}[args.leg]
+            res.update(bench_hh(ns, legs=(leg,)))
+        elif args.leg == "population":
+            res.update(bench_population(ns))
+        else:
+            scale, steps = (1.0, 300) if args.leg == "large" else (60.0, 100)
+            res.update(bench_large(ns, steps=steps, drive_scale=scale, graph=args.graph))
+        print(json.dumps(res), flush=True)
         return
 
-    spec = ns.configs.izh_stdp_network(n=args.neurons, steps=args.steps)
-    t0 = time.perf_counter()
-    eng = ns.SimulationEngine.from_spec(spec)
-    build_s = time.perf_counter() - t0
-    log(f"bench: {spec.name} n={spec.n} m={spec.m} built on device in {build_s:.2f}s")
-    if args.split:
-        eng.set_prop_split(args.split)
-    # the timed region replays captured graphs of --graph steps (the production path); a
-    # separate segment of direct launches bracketed by per-kernel hipEvents on the library's
-    # stream gives the kernel durations
-    prof = not args.no_profile
-    if args.graph:
-        eng.set_graph_steps(args.graph)
-    eng.step(args.warmup)
-    log(f"bench: warmup {args.warmup} steps done")
-    t0 = time.perf_counter()
-    st = eng.step(args.steps)
-    wall = time.perf_counter() - t0
-    log(f"bench: timed {args.steps} steps in {wall:.3f}s (device {st.wall_ms / 1e3:.3f}s)")
-    sim_s = args.steps * spec.cfg.dt_ms / 1000.0
-    events, spikes = st.events, st.spikes
-    value = events / wall
-    out = {
-        "metric": "syn events/s & x-realtime @1M/100M Izh+STDP; HH surrogate vs plain speedup",
-        "value": value,
-        "unit": "syn events/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (Philox-generated network and drive)",
-        "config": {
-            "workload": "izhikevich_pair_stdp_1M_100M",
-            "neurons": spec.n, "synapses": spec.m, "dt_ms": spec.cfg.dt_ms,
-            "v_substeps": spec.cfg.v_substeps, "delays_ms": [1, 20], "sim_ms": sim_s * 1e3,
-            "accumulate": "int64 fixed point 2^-32",
-            "launch": f"hipGraph x{args.graph}" if args.graph else "direct",
-        },
-        "x_realtime": sim_s / wall,
-        "spikes": spikes,
-        "events": events,
-        "mean_rate_hz": spikes / spec.n / sim_s,
-        "build_s": build_s,
-    }
-    if prof:
-        p_steps = min(args.steps, args.profile_steps)
-        eng.set_graph_steps(0)
-        eng.set_profiling(True)
-        pst = eng.step(p_steps)
-        eng.set_profiling(False)
-        p_events, p_spikes = pst.events, pst.spikes
-        kinds = {0: "k_update", 1: "k_synapses"}
-        times = {k: eng.kernel_time(k) for k in kinds}
-        ltp_slots, ltd_slots = times[1][2], eng.kernel_time(2)[2]
-        per = 1.0 / p_steps
-        upd_b = neuron_bytes(spec.cfg, spec.n, p_spikes * per)
-        # propagation: 4 B col + 4 B weight + 1 B delay + 8 B int64 accumulate per event and
-        # 16 B of row pointers per spiking row; potentiation: 16 B post record + 4 B weight
-        # write per E->E out-slot; depression: 4 B slot + 4 B source + 8 B pre record + 8 B
-        # weight read+write per E->E in-slot
-        prop_b = (17.0 * p_events + 16.0 * p_spikes) * per
-        syn_b = prop_b + (20.0 * ltp_slots + 24.0 * ltd_slots) * per
-        kt = {kinds[k]: {"avg_us": (ms / max(n, 1)) * 1e3, "launches": n}
-              for k, (ms, n, _) in times.items()}
-        kt["k_update"]["alg_bytes_per_launch"] = upd_b
-        kt["k_synapses"]["alg_bytes_per_launch"] = syn_b
-        kt["k_synapses"]["propagation_bytes_per_launch"] = prop_b
-        kt["k_synapses"]["events_per_launch"] = p_events * per
-        kt["k_synapses"]["ltp_slots_per_launch"] = ltp_slots * per
-        kt["k_synapses"]["ltd_slots_per_launch"] = ltd_slots * per
-        for k in kinds.values():
-            a = kt[k]["alg_bytes_per_launch"] / (max(kt[k]["avg_us"], 1e-9) * 1e-6) / 1e9
-            kt[k]["achieved_GBs"] = a
-        # the synapse pass issues one 64-bit atomic per event: rate vs the measured random
-        # atomic throughput of the device (23.0 G/s, int64 add to random addresses)
-        kt["k_synapses"]["atomics_G_per_s"] = p_events * per / (kt["k_synapses"]["avg_us"] * 1e3)
-        kt["k_synapses"]["atomic_ceiling_G_per_s"] = 23.0
-        dom = max(kinds.values(), key=lambda k: kt[k]["avg_us"])
-        out["roofline"] = {
-            "kernel": dom, "bound": "hbm", "achieved": kt[dom]["achieved_GBs"],
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kt[dom]["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom),
-        }
-        out["kernels"] = kt
-        out["profiled_segment"] = {"steps": p_steps, "launch": "direct, hipEvents per kernel",
-                                   "ms_per_step": pst.wall_ms / p_steps}
-    if not args.no_hh:
+    out, cpu_in = bench_headline(ns, args)
+    only = args.leg == "headline"
+    cpu = not args.no_cpu and not only
+    if not args.no_lif10k and not only:
+        log("bench: 10k LIF network, 1 s")
+        out["lif_10k"] = bench_lif10k(ns, cpu=cpu, budget_s=args.cpu_budget)
+    if not args.no_hh and not only:
         log("bench: HH surrogate vs plain (200k neurons)")
-        out["hh_surrogate"] = bench_hh(ns)
-        log(f"bench: HH speedup vs plain dt 0.01 ms {out['hh_surrogate']['speedup_vs_plain_dt0.01']:.2f}")
-    if not args.no_pop:
+        out["hh_surrogate"] = bench_hh(ns, cpu=cpu, budget_s=args.cpu_budget)
+        log(f"bench: HH speedup vs plain dt 0.01 ms "
+            f"{out['hh_surrogate'].get('speedup_vs_plain_dt0.01', 0):.2f}")
+    if not args.no_pop and not only:
         log("bench: batched population (256 networks)")
-        out["population"] = bench_population(ns)
+        out["population"] = bench_population(ns, cpu=cpu, budget_s=args.cpu_budget)
         log(f"bench: population {out['population']['ms_per_step']:.3f} ms/step")
-    if not args.no_large:
+    if not args.no_large and not only:
         log("bench: 10^8 LIF neurons, 10^10 synapses")
         for key, scale, steps in [("large_scale", 1.0, 300), ("large_scale_active", 60.0, 100)]:
             try:
@@ -379,8 +577,10 @@ def main():
             except ns.NsxError as e:  # recorded, not fatal: the headline line still prints
                 out[key] = {"error": str(e)}
                 log(f"bench: {key} failed: {e}")
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(spec, events / args.steps, args.cpu_steps)
+        if cpu and "error" not in out.get("large_scale", {}):
+            out["large_scale"]["cpu_baseline"] = cpu_large(ns, args.cpu_budget)
+    if cpu_in is not None and not args.no_cpu:
+        out["cpu_baseline"] = cpu_headline(cpu_in, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define NSX_ABI_VERSION 1u
+#define NSX_ABI_VERSION 2u
 
 /* GpuError {NoAdapter, DeviceRequest} (gpu_backend.rs:24-39) widened to every failure the
  * device path can report; maps onto NeuralSimError::Gpu (src/error.rs:44-46). */
@@ -160,6 +160,8 @@ typedef struct nsx_state_view {
   int32_t* post_step;
   float* post_trace2; /* triplet slow post trace                                         */
   int64_t* ring;     /* delay ring, ring_slots*channels*n_neurons                         */
+  float* izh_c;      /* Izhikevich per-neuron reset c and increment d (the jittered         */
+  float* izh_d;      /* NeuronModelParams::Izhikevich of each neuron, mod.rs:72-94)         */
 } nsx_state_view;
 
 /* ---- library ---------------------------------------------------------------------------- */
@@ -290,6 +292,16 @@ nsx_status nsx_weight_stats(nsx_ctx* ctx, double* mean, double* stddev);
 nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable);
 nsx_status nsx_kernel_time(nsx_ctx* ctx, int kind, double* total_ms, uint64_t* launches,
                            uint64_t* bytes);
+
+/* Device-side snapshot of everything a step changes: neuron state, delay ring, trace and post
+ * records, weights under plasticity, spike segments, counters and the step index (binned
+ * delivery's pending buckets are first added into the ring).  One snapshot per context; a save
+ * replaces it.  nsx_snapshot_restore returns the engine to the saved step, after which the run
+ * repeats bit for bit (drives are keyed by step).  No reference counterpart: the reference
+ * re-runs from a checkpoint file (CheckpointManager::load, src/io/checkpoint.rs:87-100); this
+ * keeps the copy in HBM, so a measurement or a tuning trial can replay the same steps. */
+nsx_status nsx_snapshot_save(nsx_ctx* ctx);
+nsx_status nsx_snapshot_restore(nsx_ctx* ctx);
 
 /* ---- GpuBackend drop-in (gpu_backend.rs), FP32, synchronous ------------------------------ */
 /* GpuBackend::new(num_neurons, num_edges) (gpu_backend.rs:98) */

@@ -74,6 +74,8 @@ PROTOTYPES = {
     "nsx_backend_step_neurons": (_int, [_VP, _P, _P, _P, _P, _P]),
     "nsx_backend_step_spmv": (_int, [_VP, _P]),
     "nsx_autotune": (_int, [_VP, _u32, C.POINTER(abi.NsxTuneReport)]),
+    "nsx_snapshot_save": (_int, [_VP]),
+    "nsx_snapshot_restore": (_int, [_VP]),
     "nsx_batch_create": (_int, [C.POINTER(abi.NsxConfig), C.POINTER(abi.NsxBatchNet), _u32,
                                 C.POINTER(_VP)]),
     "nsx_batch_layout": (_int, [_VP, _P]),

@@ -1,7 +1,7 @@
 """ctypes mirror of include/nsx.h (types and constants only; loads nothing)."""
 import ctypes as C
 
-NSX_ABI_VERSION = 1
+NSX_ABI_VERSION = 2
 
 # nsx_status
 NSX_OK = 0
@@ -219,6 +219,7 @@ class NsxStateView(C.Structure):
         ("pre_trace", _FP), ("pre_step", _IP), ("post_trace", _FP), ("post_step", _IP),
         ("post_trace2", _FP),
         ("ring", C.POINTER(C.c_int64)),
+        ("izh_c", _FP), ("izh_d", _FP),
     ]
 
 
@@ -228,6 +229,7 @@ STATE_FIELDS = {
     "syn0": "float32", "syn1": "float32", "syn2": "float32", "drive": "float32",
     "refractory": "int32", "last_spike_step": "int32", "spike_count": "uint32",
     "pre_trace": "float32", "pre_step": "int32", "post_trace": "float32", "post_step": "int32",
+    "izh_c": "float32", "izh_d": "float32",
 }
 
 

@@ -391,9 +391,11 @@ nsx_status nsx_create(const nsx_config* cfg, nsx_ctx** out) {
   }
   hipError_t e = hipSetDevice(ctx->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev_call[0]);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev_call[1]);
   if (e != hipSuccess) {
-    s = hip_error(nullptr, e, "stream creation");
-    delete ctx;
+    s = hip_error(nullptr, e, "stream / event creation");
+    nsx_destroy(ctx);
     return s;
   }
   ctx->n = cfg->n_neurons;
@@ -455,6 +457,8 @@ void nsx_destroy(nsx_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   destroy_graph(ctx);
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->ev_call)
+    if (e) (void)hipEventDestroy(e);
   for (void* p : ctx->allocs) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -592,7 +596,8 @@ nsx_status nsx_upload_state(nsx_ctx* ctx, const nsx_state_view* sv) {
       (s = copy_f(ctx, d.count, sv->spike_count, b, k)) ||
       (s = put_traces(ctx, d.pre, sv->pre_trace, sv->pre_step)) ||
       (s = put_post(ctx, sv->post_trace, sv->post_step, sv->last_spike_step)) ||
-      (s = copy_f(ctx, d.ring, sv->ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)))
+      (s = copy_f(ctx, d.ring, sv->ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)) ||
+      (s = copy_f(ctx, d.c, sv->izh_c, b, k)) || (s = copy_f(ctx, d.d, sv->izh_d, b, k)))
     return s;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if ((s = ring_mark(ctx, false))) return s;
@@ -615,7 +620,8 @@ nsx_status nsx_read_state(nsx_ctx* ctx, nsx_state_view* sv) {
       (s = copy_f(ctx, sv->spike_count, d.count, b, k)) ||
       (s = get_traces(ctx, d.pre, sv->pre_trace, sv->pre_step)) ||
       (s = get_post(ctx, sv->post_trace, sv->post_step)) ||
-      (s = copy_f(ctx, sv->ring, d.ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)))
+      (s = copy_f(ctx, sv->ring, d.ring, (size_t)ctx->D * ctx->ch * ctx->n * 8, k)) ||
+      (s = copy_f(ctx, sv->izh_c, d.c, b, k)) || (s = copy_f(ctx, sv->izh_d, d.d, b, k)))
     return s;
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return NSX_OK;
@@ -810,9 +816,8 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   uint64_t c0[kNumPart];
   nsx_status s0 = read_totals(ctx, c0);
   if (s0) return s0;
-  hipEvent_t e_beg, e_end;
-  NSX_HIP(ctx, hipEventCreate(&e_beg));
-  NSX_HIP(ctx, hipEventCreate(&e_end));
+  // the context's two call events: created once, so no return path below leaks them
+  hipEvent_t e_beg = ctx->ev_call[0], e_end = ctx->ev_call[1];
   NSX_HIP(ctx, hipEventRecord(e_beg, ctx->stream));
   const bool prof = ctx->profiling;
   const size_t nev = 5;  // events around the update, counting, synapse and arrival launches
@@ -862,8 +867,6 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
     NSX_HIP(ctx, hipMemcpy(&rast, ctx->d.counters + kCntRaster, 8, hipMemcpyDeviceToHost));
   float wall = 0.0f;
   NSX_HIP(ctx, hipEventElapsedTime(&wall, e_beg, e_end));
-  (void)hipEventDestroy(e_beg);
-  (void)hipEventDestroy(e_end);
   if (prof) {
     for (uint64_t k = first_direct; k < n_steps; ++k) {
       hipEvent_t* evs = &ctx->ev_pool[nev * k];

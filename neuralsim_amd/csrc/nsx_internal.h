@@ -233,6 +233,7 @@ struct nsx_ctx {
   double kt_ms[4] = {0, 0, 0, 0};
   uint64_t kt_launches[4] = {0, 0, 0, 0};
   std::vector<hipEvent_t> ev_pool;  // direct launches: 5 events per step
+  hipEvent_t ev_call[2] = {nullptr, nullptr};  // bracket every nsx_step call (device time)
   uint32_t last_call_steps = 0;
   std::vector<unsigned long long> part_host;
   uint64_t part_delta[4] = {0, 0, 0, 0};
@@ -255,6 +256,11 @@ struct nsx_ctx {
   uint32_t bin_te = 0;      // tile entries in use
   uint32_t update_width = 0;  // neurons per thread of the point-model update (0: by size)
   bool bin_dirty = false;  // the network changed: buffers are re-sized before the next step
+  // nsx_snapshot_save / restore: one device copy of the mutable buffers, with the step index
+  void* snap = nullptr;
+  size_t snap_bytes = 0;
+  uint64_t snap_t = 0;
+  int32_t snap_tb = 0;
   // SimulationStats kept on the host from the device's per-step counts (nsx_set_stats)
   struct {
     double rate = 0.0;         // low-pass mean firing rate (Hz)
@@ -262,6 +268,7 @@ struct nsx_ctx {
     uint64_t last = 0;         // spikes of the last step
     uint64_t steps = 0;        // steps folded
   } hs;
+  decltype(hs) snap_hs;
 };
 
 // helpers shared by translation units

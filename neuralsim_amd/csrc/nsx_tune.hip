@@ -52,11 +52,52 @@ static hipError_t copy_all(nsx_ctx* ctx, const std::vector<Buf>& bufs, char* sna
   return hipStreamSynchronize(ctx->stream);
 }
 
+static size_t snapshot_bytes(const std::vector<Buf>& bufs) {
+  size_t total = 0;
+  for (const Buf& b : bufs) total += (b.bytes + 255) & ~(size_t)255;
+  return total;
+}
+
 }  // namespace nsx
 
 using namespace nsx;
 
 extern "C" {
+
+nsx_status nsx_snapshot_save(nsx_ctx* ctx) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (!ctx->has_network) return set_error(ctx, NSX_E_STATE, "no network loaded");
+  nsx_status s = bin_materialize(ctx);  // pending buckets join the ring the snapshot holds
+  if (s) return s;
+  const std::vector<Buf> bufs = mutable_buffers(ctx);
+  const size_t total = snapshot_bytes(bufs);
+  if (ctx->snap && ctx->snap_bytes != total) {
+    dev_free(ctx, ctx->snap);
+    ctx->snap = nullptr;
+  }
+  if (!ctx->snap) {
+    if ((s = dev_alloc(ctx, &ctx->snap, total, false))) return s;
+    ctx->snap_bytes = total;
+  }
+  NSX_HIP(ctx, copy_all(ctx, bufs, (char*)ctx->snap, true));
+  ctx->snap_t = ctx->t;
+  ctx->snap_tb = ctx->tbase_host;
+  ctx->snap_hs = ctx->hs;
+  return NSX_OK;
+}
+
+nsx_status nsx_snapshot_restore(nsx_ctx* ctx) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (!ctx->snap) return set_error(ctx, NSX_E_STATE, "no snapshot saved");
+  const std::vector<Buf> bufs = mutable_buffers(ctx);
+  if (snapshot_bytes(bufs) != ctx->snap_bytes)
+    return set_error(ctx, NSX_E_STATE, "the network changed since the snapshot was saved");
+  NSX_HIP(ctx, copy_all(ctx, bufs, (char*)ctx->snap, false));
+  ctx->t = ctx->snap_t;
+  ctx->tbase_host = ctx->snap_tb;
+  ctx->hs = ctx->snap_hs;
+  return bin_clear(ctx);  // events emitted after the save are not part of it
+}
 
 nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");

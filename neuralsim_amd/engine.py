@@ -101,6 +101,13 @@ class SimulationEngine:
                   "nsx_read_csr")
         return rp, col, w, d
 
+    def row_ptr(self):
+        """The CSR row pointers alone (u64, n + 1)."""
+        n, _ = self.size()
+        rp = np.empty(n + 1, np.uint64)
+        self._chk(self._lib.nsx_read_csr(self._h, _ptr(rp), None, None, None), "nsx_read_csr")
+        return rp
+
     def weights(self):
         _, m = self.size()
         w = np.empty(m, np.float32)
@@ -205,6 +212,14 @@ class SimulationEngine:
         trials = [(r.trial_split[k], r.trial_graph[k], r.us_per_step[k], r.trial_tile[k])
                   for k in range(r.n_trials)]
         return r.prop_split, r.graph_steps, trials
+
+    def snapshot_save(self):
+        """Keep a device copy of everything a step changes (one per engine)."""
+        self._chk(self._lib.nsx_snapshot_save(self._h), "nsx_snapshot_save")
+
+    def snapshot_restore(self):
+        """Return to the saved step; the steps that follow repeat bit for bit."""
+        self._chk(self._lib.nsx_snapshot_restore(self._h), "nsx_snapshot_restore")
 
     def set_profiling(self, on):
         self._chk(self._lib.nsx_set_profiling(self._h, 1 if on else 0), "nsx_set_profiling")

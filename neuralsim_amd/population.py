@@ -29,11 +29,18 @@ class RateHomeostasis:
         return np.where(r > 500.0, f * 0.1, f)
 
 
+def lifetime_steps(lifetime_ms, dt_ms):
+    """Steps of simulate_ms(lifetime_ms): the quotient truncated, as engine.rs:792-797 casts
+    (duration_ms / dt) to usize (0.3 ms at dt 0.1 is 2.999... -> 2 steps)."""
+    q = float(lifetime_ms) / float(dt_ms)
+    return int(q) if q > 0 else 0
+
+
 def evaluate(base, nets, evaluator, graph_steps=50):
     """Run every network for evaluator.lifetime_ms in one batch; returns a dict of per-network
     spike counts, mean rates (Hz), rate standard deviation across neurons (Hz) and fitness."""
     with BatchedPopulation(base, nets) as pop:
-        steps = int(round(evaluator.lifetime_ms / base.dt_ms))
+        steps = lifetime_steps(evaluator.lifetime_ms, base.dt_ms)
         if graph_steps:
             pop.set_graph_steps(graph_steps)
         st = pop.step(steps)

@@ -1226,6 +1226,8 @@ void mir_read_state(mir_net* r, nsx_state_view* s) {
   if (s->post_trace) memcpy(s->post_trace, r->post_v, b);
   if (s->post_step) memcpy(s->post_step, r->post_s, b);
   if (s->ring) memcpy(s->ring, r->ring, (size_t)r->D * r->ch * r->n * 8);
+  if (s->izh_c) memcpy(s->izh_c, r->c, b);
+  if (s->izh_d) memcpy(s->izh_d, r->d, b);
 }
 
 void mir_write_state(mir_net* r, const nsx_state_view* s) {
@@ -1247,4 +1249,6 @@ void mir_write_state(mir_net* r, const nsx_state_view* s) {
   if (s->post_trace) memcpy(r->post_v, s->post_trace, b);
   if (s->post_step) memcpy(r->post_s, s->post_step, b);
   if (s->ring) memcpy(r->ring, s->ring, (size_t)r->D * r->ch * r->n * 8);
+  if (s->izh_c) memcpy(r->c, s->izh_c, b);
+  if (s->izh_d) memcpy(r->d, s->izh_d, b);
 }

@@ -1,6 +1,9 @@
 """Batched population: every network of the batch bit-identical to its own sequential run
 (raster, state, spike counts), with direct launches and with captured graphs, with atomic and
-binned delivery; per-network statistics exact."""
+binned delivery; per-network statistics exact; population-shaped networks bit-identical to the
+oracle's FP32 mirror (raster, state, delay ring); the full 256-network population's counters
+consistent (spikes per network, events = sum of count x out-degree, padding silent, fitness in
+(0, 1])."""
 import numpy as np
 import pytest
 
@@ -98,7 +101,8 @@ def _model_case(model):
     return base, nets, 3000, ["v", "m", "h", "n", "syn0", "syn1", "syn2", "spike_count"]
 
 
-@pytest.mark.parametrize("model,binned", [("lif", False), ("lif", True), ("hh", False)])
+@pytest.mark.parametrize("model,binned", [("lif", False), ("lif", True), ("hh", False),
+                                         ("hh", True)])
 def test_two_channel_batch_equals_sequential(nsx, model, binned):
     """Inhibitory sources feed the second ring channel by their own network's E/I split."""
     base, nets, steps, fields = _model_case(model)
@@ -126,3 +130,79 @@ def test_two_channel_batch_equals_sequential(nsx, model, binned):
     assert pst.total_spikes == total > 100
     # the inhibitory channel carried input
     assert np.any(b_state["syn1"] != 0)
+
+
+@pytest.mark.parametrize("binned", [False, True])
+def test_population_networks_match_mirror(nsx, orc, binned):
+    """Networks of the population workload (Izhikevich RS/FS with c/d jitter, 5 Hz Poisson drive,
+    delays 1-10 ms, each its own seed and in-degree) run as one batch equal the oracle's FP32
+    mirror of each network run alone: raster, state and delay ring, bit for bit."""
+    base = configs.population_base(record_raster=True, raster_capacity=1 << 22)
+    nets = configs.population_nets(5, seed=41, n_range=(257, 3000), k_range=(20, 200))
+    steps = 250
+    fields = ["v", "u", "spike_count", "last_spike_step"]
+    print(f"population vs mirror: {[b.n_neurons for b in nets]} neurons, "
+          f"{[b.k_in for b in nets]} in-degree, binned {binned}", flush=True)
+    with nsx.BatchedPopulation(base, nets) as pop:
+        if binned:
+            pop.set_delivery(nsx.abi.DELIVERY_BINNED, 0, 4096)
+        pop.set_graph_steps(50)
+        pst = pop.step(steps)
+        b_steps, b_ids = pop.raster()
+        b_state = pop.read_state(fields, ring=True)
+        off = pop.offsets.astype(np.int64)
+        N = int(off[-1])
+        net_of, local = pop.split(b_ids)
+    D = int(base.ring_slots)
+    ring = b_state["ring"].reshape(D, 1, N)
+    total_ev = 0
+    for i, net in enumerate(nets):
+        cfg = configs.population_single(base, net)
+        spec = configs.NetSpec("pop", cfg, net.k_in, (net.w_exc_lo, net.w_exc_hi),
+                               (net.w_inh_lo, net.w_inh_hi), (net.d_lo, net.d_hi), steps)
+        mir = orc.MirrorNet.from_spec(spec)
+        m_steps, m_ids, m_ev = mir.run(steps)
+        m_state = mir.read_state(fields, ring=True)
+        mir.close()
+        a, n = off[i], net.n_neurons
+        sel = net_of == i
+        assert np.array_equal(b_steps[sel], m_steps) and np.array_equal(local[sel], m_ids), i
+        for f in fields:
+            assert np.array_equal(b_state[f][a:a + n].view(np.uint32),
+                                  m_state[f].view(np.uint32)), (i, f)
+        assert np.array_equal(ring[:, :, a:a + n], m_state["ring"].reshape(D, 1, n)), i
+        total_ev += m_ev
+    assert pst.total_events == total_ev and pst.total_spikes == len(b_steps) > 500
+
+
+def test_population_256_properties(nsx):
+    """The full population workload (256 networks, 3.7 M neurons, 4.1e8 synapses), 300 steps:
+    per-network spike totals equal the sums of their neurons' counts, events equal the sum of
+    spike count x out-degree over all neurons, padding neurons never fire, every network's
+    fitness lies in (0, 1]."""
+    base = configs.population_base()
+    nets = configs.population_nets(256)
+    steps = 300
+    print(f"population 256: {sum(b.n_neurons for b in nets)} neurons, {steps} steps", flush=True)
+    with nsx.BatchedPopulation(base, nets) as pop:
+        pop.set_graph_steps(50)
+        st = pop.step(steps)
+        spikes, sq = pop.net_stats()
+        cnt = pop.read_state(["spike_count"])["spike_count"].astype(np.uint64)
+        rp = pop.row_ptr()
+        off = pop.offsets.astype(np.int64)
+    outdeg = np.diff(rp.astype(np.int64)).astype(np.uint64)
+    assert int((cnt * outdeg).sum()) == st.total_events
+    assert int(cnt.sum()) == st.total_spikes == int(spikes.sum())
+    live = np.zeros(len(cnt), bool)
+    for i, b in enumerate(nets):
+        a, n = off[i], b.n_neurons
+        assert int(cnt[a:a + n].sum()) == int(spikes[i]), i
+        assert int((cnt[a:a + n] * cnt[a:a + n]).sum()) == int(sq[i]), i
+        assert int(rp[a + n] - rp[a]) == n * b.k_in, i
+        live[a:a + n] = True
+    assert not np.any(cnt[~live]) and not np.any(outdeg[~live])
+    n = np.array([b.n_neurons for b in nets], np.float64)
+    rate = spikes / n / (steps * base.dt_ms / 1e3)
+    fit = RateHomeostasis(5.0).fitness(rate)
+    assert np.all(fit > 0) and np.all(fit <= 1) and np.count_nonzero(spikes) == len(nets)

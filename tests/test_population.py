@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from neuralsim_amd import abi, configs
-from neuralsim_amd.population import RateHomeostasis
+from neuralsim_amd.population import RateHomeostasis, lifetime_steps
 
 
 def test_rate_homeostasis_documented_values():
@@ -43,3 +43,12 @@ def test_population_networks():
     base = configs.population_base()
     assert base.model == abi.MODEL_IZHIKEVICH and base.plasticity == abi.PLAST_NONE
     assert base.ring_slots == 11 and base.drive == abi.DRIVE_POISSON
+
+
+def test_lifetime_steps_truncate_like_simulate_ms():
+    """simulate_ms runs (duration / dt) as usize steps (engine.rs:792-797): truncation."""
+    assert 0.3 / 0.1 < 3.0
+    assert lifetime_steps(0.3, 0.1) == 2
+    assert lifetime_steps(500.0, 1.0) == 500 and lifetime_steps(2000.0, 1.0) == 2000
+    assert lifetime_steps(1.0, 0.3) == 3 and lifetime_steps(0.0, 1.0) == 0
+    assert lifetime_steps(-5.0, 1.0) == 0

@@ -283,6 +283,15 @@ nsx_status nsx_set_stats(nsx_ctx* ctx, int enable);
 nsx_status nsx_read_sim_stats(nsx_ctx* ctx, nsx_sim_stats* out);
 /* Mean and population standard deviation of all synaptic weights (engine.rs:655-666). */
 nsx_status nsx_weight_stats(nsx_ctx* ctx, double* mean, double* stddev);
+/* nsx_weight_stats without any device allocation: the same result, bit for bit, with the
+ * kernels' per-block partial sums in a library-static device array (1,024 doubles = 8 KiB per
+ * GPU, held until the process ends) instead of a buffer allocated per call.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory -- a stepping loop under
+ * HIP graph capture, or a serving loop that recomputes the statistics every step as
+ * SimulationEngine::step's phase 3 does (engine.rs:655-666).  The array's need is fixed by the
+ * reduction's grid, not by the synapse count, so every network fits.  All calls on one GPU share
+ * that array: calls that overlap on one GPU race (serialise them). */
+nsx_status nsx_weight_stats_noalloc(nsx_ctx* ctx, double* mean, double* stddev);
 
 /* Kernel timing: when enabled, every launch of the named kernel class in nsx_step is bracketed
  * by hipEvents on the stream it runs on (steps run as direct launches, not graphs).  Enabling

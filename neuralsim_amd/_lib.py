@@ -62,6 +62,7 @@ PROTOTYPES = {
     "nsx_set_stats": (_int, [_VP, _int]),
     "nsx_read_sim_stats": (_int, [_VP, C.POINTER(abi.NsxSimStats)]),
     "nsx_weight_stats": (_int, [_VP, C.POINTER(_f64), C.POINTER(_f64)]),
+    "nsx_weight_stats_noalloc": (_int, [_VP, C.POINTER(_f64), C.POINTER(_f64)]),
     "nsx_set_profiling": (_int, [_VP, _int]),
     "nsx_kernel_time": (_int, [_VP, _int, C.POINTER(_f64), C.POINTER(_u64), C.POINTER(_u64)]),
     "nsx_backend_new": (_int, [_u64, _u64, C.POINTER(_VP)]),

@@ -11,6 +11,7 @@
 // on-demand two-pass reduction over all synapses with a fixed summation order.
 #include <math.h>
 
+#include <mutex>
 #include <vector>
 
 #include "nsx_internal.h"
@@ -42,6 +43,60 @@ __global__ __launch_bounds__(256) void k_weight_partials(const float* w, uint64_
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Mean and population standard deviation of the weights by two passes of kWBlocks per-block
+// partial sums into `part` (kWBlocks doubles of device memory), summed on the host in block order.
+static nsx_status weight_stats_into(nsx_ctx* ctx, double* part, double* mean, double* stddev) {
+  nsx_status s = flush_pending(ctx, true);
+  if (s) return s;
+  double h[kWBlocks];
+  double mu = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(k_weight_partials, dim3(kWBlocks), dim3(256), 0, ctx->stream, ctx->w,
+                       ctx->m, mu, pass, part);
+    NSX_HIP(ctx, hipGetLastError());
+    NSX_HIP(ctx, hipMemcpyAsync(h, part, kWBlocks * 8, hipMemcpyDeviceToHost, ctx->stream));
+    NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double a = 0.0;
+    for (uint32_t b = 0; b < kWBlocks; ++b) a += h[b];
+    if (pass == 0) mu = a / (double)ctx->m;
+    else *stddev = sqrt(a / (double)ctx->m);
+  }
+  *mean = mu;
+  return NSX_OK;
+}
+
+// Device memory of nsx_weight_stats_noalloc: the kWBlocks per-block partial sums of one pass.
+// Every block writes its own entry, so nothing needs clearing between calls.  The grid is fixed
+// at kWBlocks blocks whatever the synapse count (its loop strides), so no input needs more.  Each
+// GPU the library runs on holds one copy until the process ends; in host code the array's name
+// is not a device address -- scratch_address() gives each device's.
+constexpr uint32_t kScratchDoubles = kWBlocks;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static_assert(kWBlocks <= kScratchDoubles, "the partial sums fit the scratch array");
+constexpr int kScratchDevices = 64;
+static double* g_scratch[kScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The scratch array's address on the context's device, looked up once per device with that
+// device current (hipGetSymbolAddress) and kept.
+static nsx_status scratch_address(nsx_ctx* ctx, double** out) {
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kScratchDevices) return set_error(ctx, NSX_E_INVALID, "device index");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    int cur = -1;
+    NSX_HIP(ctx, hipGetDevice(&cur));
+    if (cur != dev) NSX_HIP(ctx, hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_error(ctx, e, "hipGetSymbolAddress(scratch)");
+    g_scratch[dev] = (double*)p;
+  }
+  *out = g_scratch[dev];
+  return NSX_OK;
 }
 
 // Fold the per-step counts of steps [first, first + n) (n <= stat_cap) into the host stats.
@@ -104,31 +159,24 @@ nsx_status nsx_weight_stats(nsx_ctx* ctx, double* mean, double* stddev) {
   *mean = 0.0;
   *stddev = 0.0;
   if (ctx->m == 0) return NSX_OK;  // the reference leaves both at 0 without synapses
-  nsx_status s = flush_pending(ctx, true);
-  if (s) return s;
   double* d = nullptr;
-  if ((s = dev_alloc(ctx, (void**)&d, kWBlocks * 8, false))) return s;
-  std::vector<double> h(kWBlocks);
-  double mu = 0.0;
-  for (int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL(k_weight_partials, dim3(kWBlocks), dim3(256), 0, ctx->stream, ctx->w,
-                       ctx->m, mu, pass, d);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(h.data(), d, kWBlocks * 8, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) {
-      dev_free(ctx, d);
-      return hip_error(ctx, e, "nsx_weight_stats");
-    }
-    double a = 0.0;
-    for (uint32_t b = 0; b < kWBlocks; ++b) a += h[b];
-    if (pass == 0) mu = a / (double)ctx->m;
-    else *stddev = sqrt(a / (double)ctx->m);
-  }
-  *mean = mu;
+  nsx_status s = dev_alloc(ctx, (void**)&d, kWBlocks * 8, false);
+  if (s) return s;
+  s = weight_stats_into(ctx, d, mean, stddev);
   dev_free(ctx, d);
-  return NSX_OK;
+  return s;
+}
+
+nsx_status nsx_weight_stats_noalloc(nsx_ctx* ctx, double* mean, double* stddev) {
+  if (!ctx || !mean || !stddev) return set_error(ctx, NSX_E_INVALID, "ctx, mean, std required");
+  if (!ctx->has_network) return set_error(ctx, NSX_E_STATE, "no network loaded");
+  *mean = 0.0;
+  *stddev = 0.0;
+  if (ctx->m == 0) return NSX_OK;
+  double* d = nullptr;
+  nsx_status s = scratch_address(ctx, &d);
+  if (s) return s;
+  return weight_stats_into(ctx, d, mean, stddev);
 }
 
 nsx_status nsx_read_sim_stats(nsx_ctx* ctx, nsx_sim_stats* out) {

@@ -228,6 +228,16 @@ nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph);
 /* Neurons per thread of the LIF / Izhikevich update: 1, 4, or 0 = by network size (4 from 2^21
  * neurons).  Results do not depend on it; HH always runs one per thread. */
 nsx_status nsx_set_update_width(nsx_ctx* ctx, uint32_t neurons_per_thread);
+/* Epoch stepping.  In a network without plasticity whose shortest delay is d_min >= 2 steps, no
+ * spike of steps t .. t + d_min - 1 arrives before step t + d_min (the reference delivers every
+ * spike at the next step, engine.rs:414-473 -- delays honoured here, with their ring).  Such
+ * steps are then advanced as one epoch: one update launch for all of them (neuron state held
+ * in registers), one synapse pass for all their spikes.  Epochs are min(d_min, max_steps) long;
+ * max_steps <= 1 turns this off, the default is 128 (at most 1024).  Binned delivery and the
+ * GpuBackend drop-in step one step at a time.  Results are bit-identical to per-step launches.
+ * nsx_get_epoch reports the epoch length the next nsx_step will use (1: per step). */
+nsx_status nsx_set_epoch(nsx_ctx* ctx, uint32_t max_steps);
+nsx_status nsx_get_epoch(nsx_ctx* ctx, uint32_t* steps);
 
 /* How propagated events reach their ring slot (the reference adds each event into
  * pending_input, engine.rs:414-494).  Results are bit-identical in both modes: the arrivals are

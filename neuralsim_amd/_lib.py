@@ -55,6 +55,8 @@ PROTOTYPES = {
     "nsx_set_graph_steps": (_int, [_VP, _u32]),
     "nsx_set_prop_split": (_int, [_VP, _u32]),
     "nsx_set_update_width": (_int, [_VP, _u32]),
+    "nsx_set_epoch": (_int, [_VP, _u32]),
+    "nsx_get_epoch": (_int, [_VP, C.POINTER(_u32)]),
     "nsx_set_delivery": (_int, [_VP, _u32, _u64, _u32]),
     "nsx_get_delivery": (_int, [_VP, C.POINTER(_u32)]),
     "nsx_read_raster": (_int, [_VP, _P, _P, _u64, C.POINTER(_u64), _int]),

@@ -710,6 +710,22 @@ nsx_status nsx_set_update_width(nsx_ctx* ctx, uint32_t neurons_per_thread) {
   return NSX_OK;
 }
 
+nsx_status nsx_set_epoch(nsx_ctx* ctx, uint32_t max_steps) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (max_steps > 1024) return set_error(ctx, NSX_E_INVALID, "epoch longer than 1024 steps");
+  ctx->epoch_cap = max_steps;
+  destroy_graph(ctx);  // captured launches hold the old grouping
+  return NSX_OK;
+}
+
+nsx_status nsx_get_epoch(nsx_ctx* ctx, uint32_t* steps) {
+  if (!ctx || !steps) return set_error(ctx, NSX_E_INVALID, "NULL argument");
+  nsx_status s = epoch_prepare(ctx);
+  if (s) return s;
+  *steps = epoch_len(ctx);
+  return NSX_OK;
+}
+
 nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (steps_per_graph > 4096) return set_error(ctx, NSX_E_INVALID, "steps_per_graph > 4096");
@@ -760,9 +776,19 @@ static nsx_status read_totals(nsx_ctx* ctx, uint64_t tot[kNumPart]) {
 static nsx_status capture_graph(nsx_ctx* ctx) {
   destroy_graph(ctx);
   const int32_t G = (int32_t)ctx->graph_steps;
+  const int32_t E = (int32_t)epoch_len(ctx);
   NSX_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   hipError_t e = hipSuccess;
-  for (int32_t k = 0; k < G && e == hipSuccess; ++k) e = enqueue_step(ctx, k, nullptr);
+  for (int32_t k = 0; k < G && e == hipSuccess;) {
+    if (E > 1) {
+      const int32_t L = std::min(E, G - k);
+      e = enqueue_epoch(ctx, k, (uint32_t)L, nullptr);
+      k += L;
+    } else {
+      e = enqueue_step(ctx, k, nullptr);
+      k += 1;
+    }
+  }
   if (e == hipSuccess) e = enqueue_advance(ctx, G);
   hipGraph_t g = nullptr;
   hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
@@ -786,6 +812,10 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
       ctx->bin_dirty = false;
       return sb;
     }
+  }
+  {
+    nsx_status se = epoch_prepare(ctx);  // the network's shortest delay, epoch segments
+    if (se) return se;
   }
   if (ctx->p.stats_on && n_steps > ctx->p.stat_cap) {
     // the per-step count ring holds stat_cap steps: run in chunks, folding after each
@@ -850,12 +880,17 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
       }
     }
   }
-  const uint64_t first_direct = done;
-  for (; done < n_steps; ++done) {
-    hipEvent_t* evs = prof ? &ctx->ev_pool[nev * done] : nullptr;
-    hipError_t e = enqueue_step(ctx, (int32_t)(ctx->t - (uint64_t)ctx->tbase_host), evs);
+  // direct launches: one step, or one epoch of up to E steps, per launch group
+  const uint64_t E = epoch_len(ctx);
+  uint64_t groups = 0;
+  for (; done < n_steps; ++groups) {
+    hipEvent_t* evs = prof ? &ctx->ev_pool[nev * groups] : nullptr;
+    const int32_t off = (int32_t)(ctx->t - (uint64_t)ctx->tbase_host);
+    const uint64_t L = E > 1 ? std::min<uint64_t>(E, n_steps - done) : 1;
+    hipError_t e = E > 1 ? enqueue_epoch(ctx, off, (uint32_t)L, evs) : enqueue_step(ctx, off, evs);
     if (e != hipSuccess) return hip_error(ctx, e, "step launch");
-    ctx->t += 1;
+    ctx->t += L;
+    done += L;
   }
   NSX_HIP(ctx, hipEventRecord(e_end, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -868,7 +903,7 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   float wall = 0.0f;
   NSX_HIP(ctx, hipEventElapsedTime(&wall, e_beg, e_end));
   if (prof) {
-    for (uint64_t k = first_direct; k < n_steps; ++k) {
+    for (uint64_t k = 0; k < groups; ++k) {
       hipEvent_t* evs = &ctx->ev_pool[nev * k];
       static const int kind_of[4] = {0, 2, 1, 3};  // event interval -> nsx_kernel_time kind
       for (size_t q = 0; q + 1 < nev; ++q) {

@@ -257,7 +257,40 @@ static nsx_status free_network(nsx_ctx* ctx) {
   return NSX_OK;
 }
 
+__global__ void k_min_delay(const uint8_t* delay, uint64_t m, unsigned int* out) {
+  unsigned int mn = 255u;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < m;
+       k += (uint64_t)gridDim.x * blockDim.x)
+    mn = min(mn, (unsigned int)delay[k]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mn = min(mn, (unsigned int)__shfl_xor((int)mn, o));
+  if ((threadIdx.x & 63u) == 0) atomicMin(out, mn);
+}
+
+nsx_status network_min_delay(nsx_ctx* ctx, uint32_t* d_min) {
+  *d_min = (uint32_t)ctx->D;
+  if (ctx->m == 0) return NSX_OK;
+  unsigned int* d = nullptr;
+  nsx_status s = dev_alloc(ctx, (void**)&d, 4, false);
+  if (s) return s;
+  const unsigned int init = 255u;
+  unsigned int h = 255u;
+  hipError_t e = hipMemcpyAsync(d, &init, 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_min_delay, dim3(blocks_for(ctx->m, 256 * 16)), dim3(256), 0,
+                       ctx->stream, ctx->delay, ctx->m, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dev_free(ctx, d);
+  if (e != hipSuccess) return hip_error(ctx, e, "minimum delay");
+  *d_min = std::min<uint32_t>(h, (uint32_t)ctx->D);
+  return NSX_OK;
+}
+
 static void publish_network(nsx_ctx* ctx) {
+  ctx->d_min = 0;  // recomputed before the next step
   ctx->d.row_ptr = ctx->row_ptr;
   ctx->d.col = ctx->col;
   ctx->d.w = ctx->w;

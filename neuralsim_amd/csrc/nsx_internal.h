@@ -113,6 +113,22 @@ struct StepParams {
   // binned delivery (bin_on = 1): tiles of 2^bin_tshift neurons, bucket (d - 1) * ntiles + tile
   uint32_t bin_on, bin_tshift, bin_ntiles, bin_nbins, bin_groups;
   uint32_t bin_cap;  // events one step's bucket buffer holds; more fall back to ring atomics
+  // epoch stepping: several steps per update launch, one synapse pass for all of them; the
+  // per-block counters are then added atomically (items of several steps share a block)
+  uint32_t part_atomic;
+};
+
+// Epoch stepping (networks without plasticity whose shortest delay is d_min >= 2 steps): no spike
+// of steps t .. t + d_min - 1 arrives before t + d_min, so those steps' updates need no
+// propagation between them.  One update launch advances all of them (state in registers, each
+// step's spikes into its own segment), then one synapse pass propagates every step's spikes.
+constexpr uint32_t kMaxEpoch = 128;  // steps per epoch at most (default cap)
+
+// Per-step spike segments of an epoch: step k's block b at spikes[(k * nblk + b) * kBlock],
+// its count at bcnt[k * nblk + b].
+struct EpochSeg {
+  uint32_t* spikes;
+  uint32_t* bcnt;
 };
 
 // per-step spike counts are added into this many stripes (one atomic per block and step)
@@ -255,6 +271,12 @@ struct nsx_ctx {
   uint32_t bin_te_req = 0;  // tile entries requested (0: automatic)
   uint32_t bin_te = 0;      // tile entries in use
   uint32_t update_width = 0;  // neurons per thread of the point-model update (0: by size)
+  // epoch stepping: steps per epoch at most (nsx_set_epoch; <= 1 off), the loaded network's
+  // shortest delay in steps (0: not known yet), the epoch segments allocated for epoch_alloc steps
+  uint32_t epoch_cap = nsx::kMaxEpoch;
+  uint32_t d_min = 0;
+  uint32_t epoch_alloc = 0;
+  nsx::EpochSeg ep{nullptr, nullptr};
   bool bin_dirty = false;  // the network changed: buffers are re-sized before the next step
   // nsx_snapshot_save / restore: one device copy of the mutable buffers, with the step index
   void* snap = nullptr;
@@ -296,6 +318,17 @@ nsx_status bin_clear(nsx_ctx* ctx);
 nsx_status ring_mark(nsx_ctx* ctx, bool never);
 // (Re)allocate the bucket buffers for the current network, or release them.
 nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t tile_entries);
+// Shortest delay (steps) of the loaded network; the ring depth when it has no synapses.
+nsx_status network_min_delay(nsx_ctx* ctx, uint32_t* d_min);
+// Steps per epoch of the context as configured now (1: per-step launches).
+uint32_t epoch_len(const nsx_ctx* ctx);
+// The loaded network's shortest delay (computed on the device after a network is installed)
+// and the epoch segments sized for epoch_len steps.
+nsx_status epoch_prepare(nsx_ctx* ctx);
+// Enqueue `len` steps from *tbase + off as one epoch (update launch + synapse pass).  Profiling
+// events as enqueue_step's.
+hipError_t enqueue_epoch(nsx_ctx* ctx, int32_t off, uint32_t len, hipEvent_t* evs);
+hipError_t launch_epoch_surrogate(nsx_ctx* ctx, int32_t off, uint32_t len);
 // The loaded network fits binned delivery with tiles of tile_entries entries.
 bool bin_supported(const nsx_ctx* ctx, uint32_t tile_entries);
 }  // namespace nsx

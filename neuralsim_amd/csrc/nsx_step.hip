@@ -687,8 +687,9 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
     for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
   }
   if (lane == 0) {
-    // parts of one block accumulate into that block's counter
-    if (P == 1) {
+    // parts of one block (or, in an epoch, several steps' items of one block) accumulate into
+    // that block's counter
+    if (P == 1 && !p.part_atomic) {
       s.part[kPartEvents * p.nblk + b] += ev;
       if (PL != NSX_PLAST_NONE) s.part[kPartLtp * p.nblk + b] += ltp;
     } else {
@@ -897,6 +898,228 @@ void k_synapses(StepParams p, DevPtrs s, int32_t off) {
     else if (PL != NSX_PLAST_NONE)
       depress_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
   });
+}
+
+// ---- epoch stepping -------------------------------------------------------------------------
+// Networks without plasticity whose shortest delay is d_min >= 2 steps: the spikes of steps
+// t .. t + d_min - 1 arrive at t + d_min at the earliest, so those steps' updates depend on no
+// propagation among them.  k_epoch_update advances `len` <= d_min steps in one launch with each
+// neuron's state in registers (one load and one store per epoch instead of per step), compacting
+// each step's spikes into that step's segment; k_synapses_epoch then propagates the spikes of all
+// `len` steps, each to ring[(t_k + d) mod D].  The arithmetic of a step is k_update's, operation
+// for operation, so an epoch run equals the per-step run bit for bit.
+template <int MODEL, int DRIVE>
+__global__ __launch_bounds__(kBlock) void k_epoch_update(StepParams p, DevPtrs s, EpochSeg ep,
+                                                         int32_t off, uint32_t len) {
+  constexpr bool TWO = MODEL != NSX_MODEL_IZHIKEVICH;
+  __shared__ uint32_t s_drive[kBlock];
+  const int32_t t0 = *s.tbase + off;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const NeuronId id = neuron_id(p, s, i, blockIdx.x);
+  const bool live = i < p.n && id.local < id.n;
+  // latest spike step before the epoch: slot t can hold input only if it is >= t - D + 1 (the
+  // epoch's own spikes arrive after it)
+  int32_t m0 = kNeverStep;
+#pragma unroll
+  for (uint32_t k = 0; k < kEvStripes; ++k) m0 = max(m0, s.ev_last[k * kEvStride]);
+  float v = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;  // LIF: v ie ii; Izh: v u; HH: v m h n
+  int32_t refr = 0, last = kNeverStep;
+  uint32_t nspk = 0;
+  HHSyn y{0.0f, 0.0f, 0.0f, 0.0f};
+  if (live) {
+    v = s.v[i];
+    if (MODEL == NSX_MODEL_LIF) {
+      w1 = s.s0[i];
+      w2 = s.s1[i];
+      refr = s.refr[i];
+    } else if (MODEL == NSX_MODEL_IZHIKEVICH) {
+      w1 = s.u[i];
+    } else {
+      w1 = s.gm[i];
+      w2 = s.gh[i];
+      w3 = s.gn[i];
+      y = hh_syn_load<DRIVE>(s, i);
+    }
+  }
+  const float v_in = v, w1_in = w1, w2_in = w2;
+  const int32_t refr_in = refr;
+  const size_t plane = (size_t)p.ch * p.n;
+  // step k's ring slot is loaded during step k - 1
+  long long qa = 0, qb = 0;
+  if (live && m0 >= t0 - (int32_t)p.D + 1) {
+    const long long* r = s.ring + (size_t)((uint32_t)t0 % p.D) * plane + i;
+    qa = r[0];
+    if (TWO) qb = r[p.n];
+  }
+  for (uint32_t k = 0; k < len; ++k) {
+    const int32_t t = t0 + (int32_t)k;
+    long long* rk = s.ring + (size_t)((uint32_t)t % p.D) * plane + i;
+    const long long a = qa, b = qb;
+    qa = 0;
+    qb = 0;
+    if (live && k + 1 < len && m0 >= t + 1 - (int32_t)p.D + 1) {
+      const long long* rn = s.ring + (size_t)((uint32_t)(t + 1) % p.D) * plane + i;
+      qa = rn[0];
+      if (TWO) qb = rn[p.n];
+    }
+    __syncthreads();  // the previous step's readers of the block's LDS are done
+    drive_stage<DRIVE>(p, s, t, s_drive, id);
+    if (DRIVE != NSX_DRIVE_NONE) __syncthreads();
+    bool spiked = false;
+    float lfp = 0.0f;
+    if (live) {
+      float a0 = dequantize(a, p.scale_out);
+      if (a != 0) rk[0] = 0;
+      float a1 = 0.0f;
+      if (TWO) {
+        a1 = dequantize(b, p.scale_out);
+        if (b != 0) rk[p.n] = 0;
+      }
+      float z = 0.0f;
+      drive_draw<DRIVE>(p, s_drive, a0, z);
+      if (MODEL == NSX_MODEL_LIF) {
+        float ie = w1 * p.dec_e;
+        ie = ie + a0;
+        float ii = w2 * p.dec_i;
+        ii = ii + a1;
+        w1 = ie;
+        w2 = ii;
+        if (refr > 0) {
+          refr = refr - 1;
+        } else {
+          float I = ie + ii;
+          float x = v - p.v_rest;
+          x = -x;
+          float yv = p.lif_r * I;
+          x = x + yv;
+          x = x * p.lif_k;
+          float vn = v + x;
+          if (vn >= p.v_th) {
+            vn = p.v_reset;
+            refr = p.refr_steps;
+            spiked = true;
+          }
+          v = vn;
+        }
+      } else if (MODEL == NSX_MODEL_IZHIKEVICH) {
+        float I = a0;
+        if (DRIVE == NSX_DRIVE_GAUSSIAN) {
+          float q = p.drive_sigma * z;
+          q = p.drive_mean + q;
+          I = I + q;
+        }
+        float u = w1;
+        if (v >= 30.0f) {
+          v = s.c[i];
+          u = u + s.d[i];
+          spiked = true;
+        } else {
+          const int ty = id.local < id.n_exc ? 0 : 1;
+          const float aa = p.izh_a[ty], bb = p.izh_b[ty];
+          const float v0 = v;
+          for (uint32_t q = 0; q < p.substeps; ++q) {
+            float t1 = 0.04f * v;
+            t1 = t1 * v;
+            float t2 = 5.0f * v;
+            float acc = t1 + t2;
+            acc = acc + 140.0f;
+            acc = acc - u;
+            acc = acc + I;
+            acc = acc * p.izh_h;
+            v = v + acc;
+          }
+          float t3 = bb * v0;
+          t3 = t3 - u;
+          t3 = aa * t3;
+          t3 = t3 * p.dtf;
+          u = u + t3;
+        }
+        w1 = u;
+      } else {
+        const float I = hh_input<DRIVE>(p, y, v, a0, a1, z, lfp);
+        float vn = v;
+        for (uint32_t q = 0; q < p.hh_substeps; ++q) hh_substep(vn, w1, w2, w3, I, p.hh);
+        if (p.hh_reset_mode) {
+          spiked = vn >= -55.0f;
+          if (spiked) vn = -70.0f;
+        } else {
+          spiked = (v < p.hh_spike_v) && (vn >= p.hh_spike_v);
+        }
+        v = vn;
+      }
+      if (spiked) {
+        nspk += 1u;
+        last = t;
+      }
+    }
+    if (MODEL == NSX_MODEL_HH && p.stats_on) lfp_accumulate(s, lfp, blockIdx.x);
+    DevPtrs sk = s;
+    sk.spikes = ep.spikes + ((size_t)k * p.nblk) * kBlock;
+    sk.bcnt = ep.bcnt + (size_t)k * p.nblk;
+    block_compact(p, sk, spiked, i, t, blockIdx.x);
+  }
+  if (live) {
+    if (MODEL == NSX_MODEL_LIF) {  // unchanged values are not rewritten, as in k_update
+      if (__float_as_uint(w1) != __float_as_uint(w1_in)) s.s0[i] = w1;
+      if (__float_as_uint(w2) != __float_as_uint(w2_in)) s.s1[i] = w2;
+      if (refr != refr_in) s.refr[i] = refr;
+      if (__float_as_uint(v) != __float_as_uint(v_in)) s.v[i] = v;
+    } else if (MODEL == NSX_MODEL_IZHIKEVICH) {
+      s.v[i] = v;
+      s.u[i] = w1;
+    } else {
+      s.v[i] = v;
+      s.gm[i] = w1;
+      s.gh[i] = w2;
+      s.gn[i] = w3;
+      hh_syn_store<DRIVE>(s, i, y);
+    }
+    if (nspk) {
+      s.count[i] += nspk;
+      s.last[i] = last;
+    }
+  }
+}
+
+// The synapse pass of an epoch: propagation items (step k, block b, part q) of every step,
+// handed to the wavefronts as for_items does for one step.
+template <int PL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
+void k_synapses_epoch(StepParams p, DevPtrs s, EpochSeg ep, int32_t off, uint32_t len) {
+  __shared__ RowChunk sh[kBlock / 64];
+  const int32_t t0 = *s.tbase + off;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const uint32_t P = p.prop_split;
+  const uint64_t per = (uint64_t)p.nblk * P;
+  const uint64_t nwork = per * len;
+  const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+  for (uint64_t base = wave; base < nwork; base += 64ull * nwaves) {
+    const uint64_t mine = base + (uint64_t)lane * nwaves;
+    uint32_t c = 0, qq = 0;
+    if (mine < nwork) {
+      const uint32_t kk = (uint32_t)(mine / per);
+      const uint32_t rem = (uint32_t)(mine - (uint64_t)kk * per);
+      const uint32_t bb = rem / P;
+      qq = rem - bb * P;
+      c = ep.bcnt[(size_t)kk * p.nblk + bb];
+    }
+    unsigned long long act = __ballot(mine < nwork && c > qq);
+    while (act) {
+      const uint32_t l = (uint32_t)(__ffsll((long long)act) - 1);
+      act &= act - 1ull;
+      const uint64_t item = base + (uint64_t)l * nwaves;
+      const uint32_t kk = __builtin_amdgcn_readfirstlane((uint32_t)(item / per));
+      const uint32_t rem = (uint32_t)(item - (uint64_t)kk * per);
+      const uint32_t b = __builtin_amdgcn_readfirstlane(rem / P);
+      const uint32_t q = rem - b * P;
+      const uint32_t cnt = __builtin_amdgcn_readfirstlane(__shfl(c, (int)l));
+      DevPtrs sk = s;
+      sk.spikes = ep.spikes + ((size_t)kk * p.nblk) * kBlock;
+      propagate_item<PL, false>(p, sk, sh[wid], t0 + (int32_t)kk, b, q, cnt, lane, nullptr,
+                                nullptr);
+    }
+  }
 }
 
 // ---- binned delivery ------------------------------------------------------------------------
@@ -1364,6 +1587,84 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
     if (evs) (void)hipEventRecord(evs[3], ctx->stream);
   }
   if (evs) (void)hipEventRecord(evs[4], ctx->stream);
+  return hipGetLastError();
+}
+
+typedef void (*EpochFn)(StepParams, DevPtrs, EpochSeg, int32_t, uint32_t);
+
+template <int MODEL>
+static EpochFn epoch_pick_drive(int drive) {
+  switch (drive) {
+    case NSX_DRIVE_POISSON: return k_epoch_update<MODEL, NSX_DRIVE_POISSON>;
+    case NSX_DRIVE_GAUSSIAN: return k_epoch_update<MODEL, NSX_DRIVE_GAUSSIAN>;
+    case NSX_DRIVE_OU: return k_epoch_update<MODEL, NSX_DRIVE_OU>;
+    default: return k_epoch_update<MODEL, NSX_DRIVE_NONE>;
+  }
+}
+
+static EpochFn pick_epoch(int model, int drive) {
+  switch (model) {
+    case NSX_MODEL_LIF: return epoch_pick_drive<NSX_MODEL_LIF>(drive);
+    case NSX_MODEL_IZHIKEVICH: return epoch_pick_drive<NSX_MODEL_IZHIKEVICH>(drive);
+    default: return epoch_pick_drive<NSX_MODEL_HH>(drive);
+  }
+}
+
+uint32_t epoch_len(const nsx_ctx* ctx) {
+  if (ctx->epoch_cap < 2 || ctx->cfg.plasticity != NSX_PLAST_NONE || ctx->backend_mode ||
+      ctx->p.bin_on || ctx->delivery != NSX_DELIVERY_ATOMIC || ctx->d_min < 2)
+    return 1;
+  return std::min(ctx->d_min, ctx->epoch_cap);
+}
+
+nsx_status epoch_prepare(nsx_ctx* ctx) {
+  if (ctx->has_network && ctx->d_min == 0) {
+    nsx_status s = network_min_delay(ctx, &ctx->d_min);
+    if (s) return s;
+  }
+  const uint32_t E = epoch_len(ctx);
+  const uint32_t atomic = E > 1 ? 1u : 0u;
+  if (atomic != ctx->p.part_atomic) {
+    ctx->p.part_atomic = atomic;
+    destroy_graph(ctx);  // captured launches hold the old parameters
+  }
+  if (E > 1 && ctx->epoch_alloc < E) {
+    dev_free(ctx, ctx->ep.spikes);
+    dev_free(ctx, ctx->ep.bcnt);
+    ctx->ep = EpochSeg{nullptr, nullptr};
+    ctx->epoch_alloc = 0;
+    destroy_graph(ctx);
+    nsx_status s;
+    if ((s = dev_alloc(ctx, (void**)&ctx->ep.spikes, (size_t)E * ctx->p.nblk * kBlock * 4, false)) ||
+        (s = dev_alloc(ctx, (void**)&ctx->ep.bcnt, (size_t)E * ctx->p.nblk * 4, true)))
+      return s;
+    ctx->epoch_alloc = E;
+  }
+  return NSX_OK;
+}
+
+hipError_t enqueue_epoch(nsx_ctx* ctx, int32_t off, uint32_t len, hipEvent_t* evs) {
+  const StepParams& p = ctx->p;
+  if (evs) (void)hipEventRecord(evs[0], ctx->stream);
+  if (ctx->cfg.model == NSX_MODEL_HH_SURROGATE) {
+    const hipError_t e = launch_epoch_surrogate(ctx, off, len);
+    if (e != hipSuccess) return e;
+  } else {
+    hipLaunchKernelGGL(pick_epoch((int)ctx->cfg.model, (int)ctx->cfg.drive), dim3(p.nblk),
+                       dim3(kBlock), 0, ctx->stream, p, ctx->d, ctx->ep, off, len);
+  }
+  if (evs) {
+    (void)hipEventRecord(evs[1], ctx->stream);
+    (void)hipEventRecord(evs[2], ctx->stream);
+  }
+  const uint64_t waves = (uint64_t)len * p.nblk * p.prop_split;
+  const uint32_t gs = (uint32_t)std::min<uint64_t>((waves + kBlock / 64 - 1) / (kBlock / 64), 16384);
+  hipLaunchKernelGGL(k_synapses_epoch<NSX_PLAST_NONE>, dim3(std::max(gs, 1u)), dim3(kBlock), 0,
+                     ctx->stream, p, ctx->d, ctx->ep, off, len);
+  if (evs) {
+    (void)hipEventRecord(evs[3], ctx->stream);
+    (void)hipEventRecord(evs[4], ctx->stream);
+  }
   return hipGetLastError();
 }
 

@@ -483,6 +483,136 @@ hipError_t launch_update_surrogate(nsx_ctx* ctx, int32_t off) {
   return hipGetLastError();
 }
 
+// Epoch stepping (networks without plasticity, shortest delay d_min >= 2 steps): `len` steps of
+// k_update_surr in one launch, each neuron's (V, m, h, n), conductances and OU value in registers,
+// the MLP's scalars staged once, each step's spikes into its own segment (k_synapses_epoch
+// propagates them).  Every step is k_update_surr's arithmetic, so the run is bit-identical.
+template <int DRIVE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void k_epoch_surr(StepParams p, DevPtrs s, SurrDev q,
+                                                       EpochSeg ep, int32_t off, uint32_t len) {
+  __shared__ InferLds L[kBlock / 64];
+  __shared__ MlpScalars S;
+  __shared__ uint32_t s_fb[kBlock / 64];
+  __shared__ uint32_t s_drive[kBlock];
+  const int32_t t0 = *s.tbase + off;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  const NeuronId id = neuron_id(p, s, i, blockIdx.x);
+  const bool live = i < p.n && id.local < id.n;
+  int32_t m0 = kNeverStep;
+#pragma unroll
+  for (uint32_t k = 0; k < kEvStripes; ++k) m0 = max(m0, s.ev_last[k * kEvStride]);
+  HHSyn y{0.0f, 0.0f, 0.0f, 0.0f};
+  float st[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // V, m, h, n
+  if (live) {
+    st[0] = s.v[i];
+    st[1] = s.gm[i];
+    st[2] = s.gh[i];
+    st[3] = s.gn[i];
+    y = hh_syn_load<DRIVE>(s, i);
+  }
+  stage_scalars(S, q.img, q.norm);
+  const size_t plane = (size_t)p.ch * p.n;
+  long long qa = 0, qb = 0;
+  if (live && m0 >= t0 - (int32_t)p.D + 1) {
+    const long long* r = s.ring + (size_t)((uint32_t)t0 % p.D) * plane + i;
+    qa = r[0];
+    qb = r[p.n];
+  }
+  uint32_t nspk = 0, wave_fb = 0;
+  int32_t last = kNeverStep;
+  for (uint32_t k = 0; k < len; ++k) {
+    const int32_t t = t0 + (int32_t)k;
+    long long* rk = s.ring + (size_t)((uint32_t)t % p.D) * plane + i;
+    const long long a = qa, b = qb;
+    qa = 0;
+    qb = 0;
+    if (live && k + 1 < len && m0 >= t + 1 - (int32_t)p.D + 1) {
+      const long long* rn = s.ring + (size_t)((uint32_t)(t + 1) % p.D) * plane + i;
+      qa = rn[0];
+      qb = rn[p.n];
+    }
+    __syncthreads();  // the previous step's readers of the block's LDS are done
+    drive_stage<DRIVE>(p, s, t, s_drive, id);
+    __syncthreads();
+    const MlpNorm& nm = S.norm;
+    float lfp = 0.0f;
+    float x[kMlpIn];
+    if (live) {
+      float a0 = dequantize(a, p.scale_out);
+      if (a != 0) rk[0] = 0;
+      float a1 = dequantize(b, p.scale_out);
+      if (b != 0) rk[p.n] = 0;
+      float z = 0.0f;
+      drive_draw<DRIVE>(p, s_drive, a0, z);
+      x[4] = hh_input<DRIVE>(p, y, st[0], a0, a1, z, lfp);
+      x[0] = st[0];
+      x[1] = st[1];
+      x[2] = st[2];
+      x[3] = st[3];
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < kMlpIn; ++kk) x[kk] = nm.in_mu[kk];
+    }
+    float nx[4], res[kMlpOut];
+    const bool in = surr_step(*q.img, S, L[wid], p.hh, p.hh_substeps, x, live, nx, res, lane);
+    bool spiked = false;
+    if (live) {
+      spiked = (x[0] < p.hh_spike_v) && (nx[0] >= p.hh_spike_v);
+      st[0] = nx[0];
+      st[1] = nx[1];
+      st[2] = nx[2];
+      st[3] = nx[3];
+      if (spiked) {
+        nspk += 1u;
+        last = t;
+      }
+    }
+    if (p.stats_on) lfp_accumulate(s, lfp, blockIdx.x);
+    wave_fb += (uint32_t)__popcll(__ballot(live && !in));
+    DevPtrs sk = s;
+    sk.spikes = ep.spikes + ((size_t)k * p.nblk) * kBlock;
+    sk.bcnt = ep.bcnt + (size_t)k * p.nblk;
+    block_compact(p, sk, spiked, i, t, blockIdx.x);
+  }
+  if (live) {
+    s.v[i] = st[0];
+    s.gm[i] = st[1];
+    s.gh[i] = st[2];
+    s.gn[i] = st[3];
+    hh_syn_store<DRIVE>(s, i, y);
+    if (nspk) {
+      s.count[i] += nspk;
+      s.last[i] = last;
+    }
+  }
+  if (lane == 0) s_fb[wid] = wave_fb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t cnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBlock / 64; ++k) cnt += s_fb[k];
+    if (cnt) q.fb[blockIdx.x] += cnt;
+  }
+}
+
+typedef void (*SurrEpochFn)(StepParams, DevPtrs, SurrDev, EpochSeg, int32_t, uint32_t);
+
+hipError_t launch_epoch_surrogate(nsx_ctx* ctx, int32_t off, uint32_t len) {
+  SurrEpochFn f;
+  switch (ctx->cfg.drive) {
+    case NSX_DRIVE_POISSON: f = k_epoch_surr<NSX_DRIVE_POISSON>; break;
+    case NSX_DRIVE_GAUSSIAN: f = k_epoch_surr<NSX_DRIVE_GAUSSIAN>; break;
+    case NSX_DRIVE_OU: f = k_epoch_surr<NSX_DRIVE_OU>; break;
+    default: f = k_epoch_surr<NSX_DRIVE_NONE>; break;
+  }
+  const SurrDev q{(const MlpImage*)ctx->sur.image, (const MlpNorm*)ctx->sur.norm, ctx->sur.fb};
+  hipLaunchKernelGGL(f, dim3(ctx->p.nblk), dim3(kBlock), 0, ctx->stream, ctx->p, ctx->d, q,
+                     ctx->ep, off, len);
+  return hipGetLastError();
+}
+
 // ---- host side ------------------------------------------------------------------------------
 
 static nsx_status need_hh(nsx_ctx* ctx) {

@@ -188,6 +188,17 @@ class SimulationEngine:
         self._chk(self._lib.nsx_set_update_width(self._h, int(neurons_per_thread)),
                   "nsx_set_update_width")
 
+    def set_epoch(self, max_steps):
+        """Steps per epoch at most (<= 1: one step per launch)."""
+        self._chk(self._lib.nsx_set_epoch(self._h, max_steps), "nsx_set_epoch")
+
+    @property
+    def epoch(self):
+        """Steps per epoch the next step() will use (1: per-step launches)."""
+        e = C.c_uint32()
+        self._chk(self._lib.nsx_get_epoch(self._h, C.byref(e)), "nsx_get_epoch")
+        return e.value
+
     def set_delivery(self, mode, cap_events=0, tile_entries=0):
         """How events reach the ring (abi.DELIVERY_ATOMIC / DELIVERY_BINNED); results are
         bit-identical.  cap_events bounds one step's bucket buffer (more events: atomics that

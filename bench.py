@@ -371,7 +371,11 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
         build_s = time.perf_counter() - t0
         pop.set_graph_steps(10)
         pop.step(warm)
-        split, graph, trials = pop.autotune(tune_steps)
+        split, graph, _ = pop.autotune(tune_steps)
+        r = pop.last_tune
+        row_split, syn_block, width = int(r.row_split), int(r.syn_block), int(r.update_width)
+        trials = [{k: (round(v, 1) if k == "us_per_step" else int(v)) for k, v in t.items()}
+                  for t in pop.tune_trials()]
         t0 = time.perf_counter()
         st = pop.step(steps - warm)
         wall = time.perf_counter() - t0
@@ -385,8 +389,8 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
            "steps": steps, "build_s": build_s,
            "tuned": {"prop_split": split, "graph_steps": graph,
                      "delivery": "binned" if deliv else "atomic", "tile_entries": tile,
-                     "trials_us_per_step": [[int(a), int(b), round(c, 1), int(d)]
-                                            for a, b, c, d in trials]},
+                     "row_split": row_split, "syn_block": syn_block, "update_width": width,
+                     "trials": trials},
            "timed_steps": steps - warm,
            "ms_per_step": wall * 1e3 / (steps - warm),
            "x_realtime": (steps - warm) * base.dt_ms / 1e3 / wall,
@@ -484,6 +488,34 @@ def cpu_large(ns, budget_s):
         {"ms_per_step": s_step * 1e3, "scale": 0.01})
 
 
+def probe_atomics(reps=20):
+    """Device-wide throughput of random integer atomic adds: 2^24 adds per launch to uniformly
+    random entries of a 168 MB int64 array (the 1M network's ring: 21 slots x 10^6 neurons) and
+    of an 84 MB int32 array, through torch's scatter_add_ (one hardware atomic per element; its
+    index and value streams are coalesced).  Returns G atomics per second."""
+    import torch
+    dev = torch.device("cuda:0")
+    n_adds, out = 1 << 24, {}
+    gen = torch.Generator(device=dev).manual_seed(7)
+    for name, dtype in (("int64", torch.int64), ("int32", torch.int32)):
+        tgt = torch.zeros(21 * 1_000_000, dtype=dtype, device=dev)
+        idx = torch.randint(0, tgt.numel(), (n_adds,), generator=gen, device=dev)
+        val = torch.ones(n_adds, dtype=dtype, device=dev)
+        tgt.scatter_add_(0, idx, val)  # warm up
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            tgt.scatter_add_(0, idx, val)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        assert int(tgt.sum().item()) == (reps + 1) * n_adds
+        out[name] = {"adds_per_launch": n_adds, "target_bytes": tgt.numel() * tgt.element_size(),
+                     "us_per_launch": ms * 1e3, "G_atomics_per_s": n_adds / (ms * 1e-3) / 1e9}
+    return out
+
+
 LEGS = ("headline", "lif10k", "hh_surrogate", "hh_plain", "hh_plain_x5", "population",
         "large", "large_active")
 
@@ -511,6 +543,8 @@ def main():
     ap.add_argument("--no-lif10k", action="store_true", help="skip the 10k LIF network")
     ap.add_argument("--leg", choices=LEGS,
                     help="run only this workload (one engine per process, for kernel traces)")
+    ap.add_argument("--probe-atomics", action="store_true",
+                    help="measure the device's random int64 / int32 atomic-add throughput only")
     ap.add_argument("--counters-window", action="store_true",
                     help="headline network only: burn-in, warm-up and the timed steps as direct "
                          "launches, nothing else (for rocprofv3 --pmc over that window)")
@@ -520,6 +554,9 @@ def main():
               "GPUs are not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.probe_atomics:
+        print(json.dumps({"atomics": probe_atomics()}), flush=True)
+        return
     import neuralsim_amd as ns
     if args.counters_window:
         spec = ns.configs.izh_stdp_network(n=args.neurons)

@@ -228,6 +228,14 @@ nsx_status nsx_set_graph_steps(nsx_ctx* ctx, uint32_t steps_per_graph);
 /* Neurons per thread of the LIF / Izhikevich update: 1, 4, or 0 = by network size (4 from 2^21
  * neurons).  Results do not depend on it; HH always runs one per thread. */
 nsx_status nsx_set_update_width(nsx_ctx* ctx, uint32_t neurons_per_thread);
+/* High-degree rows: with atomic delivery, a spiking row longer than `slots` synapses is cut into
+ * pieces of at most `slots` that the block's wavefronts share (nsx_set_prop_split), so one hub
+ * neuron's row is walked by several wavefronts.  0 (default) keeps whole rows; else 16 .. 2^20.
+ * Results do not depend on it. */
+nsx_status nsx_set_row_split(nsx_ctx* ctx, uint32_t slots);
+/* Threads per workgroup of the atomic-delivery synapse pass: 64, 128, 256 (default) or 512.
+ * Results do not depend on it. */
+nsx_status nsx_set_syn_block(nsx_ctx* ctx, uint32_t threads);
 /* Epoch stepping.  In a network without plasticity whose shortest delay is d_min >= 2 steps, no
  * spike of steps t .. t + d_min - 1 arrives before step t + d_min (the reference delivers every
  * spike at the next step, engine.rs:414-473 -- delays honoured here, with their ring).  Such
@@ -359,19 +367,25 @@ nsx_status nsx_backend_step_spmv(nsx_ctx* ctx, float* input_current);
 /* ---- launch-shape autotuner --------------------------------------------------------------------
  * No reference counterpart (the reference has one fixed wgpu dispatch shape, gpu_backend.rs:
  * 921-981).  Times trial_steps steps (rounded up to a multiple of 200, the same steps in every
- * trial) for each candidate propagation split (wavefronts sharing a
- * neuron block's spike segment: 1, 2, 4, 8, 16) at a 50-step graph window, then each graph window
- * (10, 25, 100, 200 steps) at the best split, then binned delivery with 8192- and 16384-entry
- * tiles where the network allows it, every trial from a device-side snapshot of the mutable
- * state; restores the state and the step index, and sets the fastest combination. */
+ * trial) for each candidate, in a coordinate search: the propagation split (wavefronts sharing
+ * a neuron block's spike segment: 1, 2, 4, 8, 16) at a 50-step graph window; under atomic
+ * delivery the row-split threshold for high-degree rows (off, 256, 128, 64 slots) and the
+ * synapse pass's workgroup size (128, 256, 512 threads); for LIF / Izhikevich the update's
+ * neurons per thread (1, 4); the graph window (10, 25, 100, 200 steps); the other deliveries
+ * (atomic, binned with 8192- and 16384-entry tiles where the network allows it) and, when the
+ * delivery changed, the split again.  Every trial starts from a device-side snapshot of the
+ * mutable state; the state and the step index are restored and the fastest combination set. */
 typedef struct nsx_tune_report {
   uint32_t prop_split, graph_steps; /* chosen                                                 */
   uint32_t n_trials, _pad;
   double best_us_per_step;
-  double us_per_step[32];           /* per trial, in trial order                              */
-  uint32_t trial_split[32], trial_graph[32];
+  double us_per_step[48];           /* per trial, in trial order                              */
+  uint32_t trial_split[48], trial_graph[48];
   uint32_t delivery, tile_entries;  /* chosen (tile_entries 0 with atomic delivery)           */
-  uint32_t trial_tile[32];          /* per trial: 0 atomic delivery, else binned tile entries */
+  uint32_t trial_tile[48];          /* per trial: 0 atomic delivery, else binned tile entries */
+  uint32_t row_split, syn_block;    /* chosen row-split threshold (0: whole rows), workgroup  */
+  uint32_t update_width, _pad2;     /* chosen neurons per update thread (0: not a point model) */
+  uint32_t trial_row_split[48], trial_syn_block[48], trial_width[48];
 } nsx_tune_report;
 
 nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* report);

@@ -56,6 +56,8 @@ PROTOTYPES = {
     "nsx_set_prop_split": (_int, [_VP, _u32]),
     "nsx_set_update_width": (_int, [_VP, _u32]),
     "nsx_set_epoch": (_int, [_VP, _u32]),
+    "nsx_set_row_split": (_int, [_VP, _u32]),
+    "nsx_set_syn_block": (_int, [_VP, _u32]),
     "nsx_get_epoch": (_int, [_VP, C.POINTER(_u32)]),
     "nsx_set_delivery": (_int, [_VP, _u32, _u64, _u32]),
     "nsx_get_delivery": (_int, [_VP, C.POINTER(_u32)]),

@@ -137,12 +137,19 @@ class NsxTuneReport(C.Structure):
         ("n_trials", C.c_uint32),
         ("_pad", C.c_uint32),
         ("best_us_per_step", C.c_double),
-        ("us_per_step", C.c_double * 32),
-        ("trial_split", C.c_uint32 * 32),
-        ("trial_graph", C.c_uint32 * 32),
+        ("us_per_step", C.c_double * 48),
+        ("trial_split", C.c_uint32 * 48),
+        ("trial_graph", C.c_uint32 * 48),
         ("delivery", C.c_uint32),
         ("tile_entries", C.c_uint32),
-        ("trial_tile", C.c_uint32 * 32),
+        ("trial_tile", C.c_uint32 * 48),
+        ("row_split", C.c_uint32),
+        ("syn_block", C.c_uint32),
+        ("update_width", C.c_uint32),
+        ("_pad2", C.c_uint32),
+        ("trial_row_split", C.c_uint32 * 48),
+        ("trial_syn_block", C.c_uint32 * 48),
+        ("trial_width", C.c_uint32 * 48),
     ]
 
 

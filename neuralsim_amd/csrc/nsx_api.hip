@@ -710,6 +710,24 @@ nsx_status nsx_set_update_width(nsx_ctx* ctx, uint32_t neurons_per_thread) {
   return NSX_OK;
 }
 
+nsx_status nsx_set_row_split(nsx_ctx* ctx, uint32_t slots) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (slots != 0 && (slots < 16 || slots > (1u << 20)))
+    return set_error(ctx, NSX_E_INVALID, "row split threshold must be 0 or 16 .. 2^20 slots");
+  ctx->p.row_split = slots;
+  destroy_graph(ctx);  // captured launches hold the old parameters
+  return NSX_OK;
+}
+
+nsx_status nsx_set_syn_block(nsx_ctx* ctx, uint32_t threads) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (threads != 64 && threads != 128 && threads != 256 && threads != 512)
+    return set_error(ctx, NSX_E_INVALID, "synapse workgroup size must be 64, 128, 256 or 512");
+  ctx->syn_block = threads;
+  destroy_graph(ctx);
+  return NSX_OK;
+}
+
 nsx_status nsx_set_epoch(nsx_ctx* ctx, uint32_t max_steps) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (max_steps > 1024) return set_error(ctx, NSX_E_INVALID, "epoch longer than 1024 steps");

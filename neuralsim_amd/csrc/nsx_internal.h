@@ -116,6 +116,9 @@ struct StepParams {
   // epoch stepping: several steps per update launch, one synapse pass for all of them; the
   // per-block counters are then added atomically (items of several steps share a block)
   uint32_t part_atomic;
+  // propagation with atomic delivery: rows longer than row_split slots are cut into pieces of at
+  // most row_split slots, dealt round-robin to the block's prop_split wavefronts (0: whole rows)
+  uint32_t row_split;
 };
 
 // Epoch stepping (networks without plasticity whose shortest delay is d_min >= 2 steps): no spike
@@ -271,6 +274,7 @@ struct nsx_ctx {
   uint32_t bin_te_req = 0;  // tile entries requested (0: automatic)
   uint32_t bin_te = 0;      // tile entries in use
   uint32_t update_width = 0;  // neurons per thread of the point-model update (0: by size)
+  uint32_t syn_block = 256;   // threads per workgroup of the (atomic-delivery) synapse pass
   // epoch stepping: steps per epoch at most (nsx_set_epoch; <= 1 off), the loaded network's
   // shortest delay in steps (0: not known yet), the epoch segments allocated for epoch_alloc steps
   uint32_t epoch_cap = nsx::kMaxEpoch;

@@ -650,14 +650,104 @@ __device__ unsigned long long propagate_chunk(const StepParams& p, const DevPtrs
   return ltp;
 }
 
+// Per-block counters of a propagation item: events and potentiated slots of this wavefront.
+template <int PL>
+__device__ inline void item_counters(const StepParams& p, const DevPtrs& s, uint32_t b,
+                                     unsigned long long ev, unsigned long long ltp,
+                                     uint32_t lane) {
+  if (PL != NSX_PLAST_NONE) {
+#pragma unroll
+    for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
+  }
+  if (lane == 0) {
+    // parts of one block (or, in an epoch, several steps' items of one block) accumulate into
+    // that block's counter
+    if (p.prop_split == 1 && !p.part_atomic) {
+      s.part[kPartEvents * p.nblk + b] += ev;
+      if (PL != NSX_PLAST_NONE) s.part[kPartLtp * p.nblk + b] += ltp;
+    } else {
+      atomicAdd(&s.part[kPartEvents * p.nblk + b], ev);
+      if (PL != NSX_PLAST_NONE) atomicAdd(&s.part[kPartLtp * p.nblk + b], ltp);
+    }
+  }
+}
+
+// Propagation with a row-split threshold T = p.row_split: the spiking rows of block b are cut
+// into pieces of at most T slots (a row of d slots into ceil(d / T)), the pieces numbered in row
+// order, and part q of P takes pieces q, q + P, ...  A high-degree row is then walked by several
+// wavefronts instead of one.  Each slot is still delivered and, under STDP, updated exactly once.
+template <int PL>
+__device__ void propagate_pieces(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
+                                 uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane) {
+  const uint32_t P = p.prop_split, T = p.row_split;
+  unsigned long long ev = 0, ltp = 0;
+  uint32_t before = 0;  // pieces of the rows of earlier row chunks
+  for (uint32_t r0 = 0; r0 < cnt; r0 += 64) {
+    const uint32_t nr = min(64u, cnt - r0);
+    uint32_t i = 0, deg = 0;
+    if (lane < nr) {
+      i = s.spikes[b * kBlock + r0 + lane];
+      deg = (uint32_t)(s.row_ptr[i + 1] - s.row_ptr[i]);
+    }
+    uint32_t incl = (deg + T - 1) / T;  // pieces of this lane's row, then their prefix
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    const uint32_t tot = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
+    rc.it_end[lane] = incl;
+    rc.it_base[lane] = i;
+    wave_sync();
+    // this part's pieces of the chunk: g with (before + g) mod P == q
+    for (uint32_t g0 = (q + P - before % P) % P; g0 < tot; g0 += 64 * P) {
+      const uint32_t g = g0 + lane * P;
+      uint32_t pi = 0, pdeg = 0;
+      uint64_t pbeg = 0;
+      if (g < tot) {
+        uint32_t r = 0;
+#pragma unroll
+        for (uint32_t st = 32; st > 0; st >>= 1)
+          if (rc.it_end[r + st - 1] <= g) r += st;
+        const uint32_t k = g - (r ? rc.it_end[r - 1] : 0u);  // piece k of row r
+        pi = rc.it_base[r];
+        const uint64_t rb = s.row_ptr[pi];
+        const uint32_t rd = (uint32_t)(s.row_ptr[pi + 1] - rb);
+        pbeg = rb + (uint64_t)k * T;
+        pdeg = min(T, rd - k * T);
+        if (p.ch > 1) rc.chan[lane] = source_channel(p, s, pi);
+        if (PL == NSX_PLAST_PAIR && pi < p.n_exc) {
+          const TraceRec a = s.pre[pi];
+          rc.pre_v[lane] = a.v;
+          rc.pre_vp[lane] = a.v_prev;
+          rc.pre_sp[lane] = a.s_prev;
+        }
+      }
+      const uint32_t total = chunk_setup(rc, lane, pdeg, pbeg, pi);
+      ev += total;
+      ltp += propagate_chunk<PL, false>(p, s, rc, t, total, lane, nullptr, nullptr);
+      wave_sync();
+    }
+    before += tot;
+    wave_sync();  // it_end / it_base are rewritten by the next row chunk
+  }
+  item_counters<PL>(p, s, b, ev, ltp, lane);
+}
+
 // Propagation work item: part q of neuron block b.  PL: STDP applied on the spiking rows' E->E
 // slots -- pair: the target's pending depressions, potentiation by the post trace (then
 // depression if the target fired at t); triplet: depression by the target's last spike.
-template <int PL, bool BIN>
+// PIECES: the row-split instance (p.row_split > 0, atomic delivery); a separate kernel instance
+// keeps the whole-row path's registers unchanged.
+template <int PL, bool BIN, bool PIECES = false>
 __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& rc, int32_t t,
                                uint32_t b, uint32_t q, uint32_t cnt, uint32_t lane,
                                uint32_t* pos, uint2* bev) {
   const uint32_t P = p.prop_split;
+  if constexpr (PIECES && !BIN) {
+    if (cnt) propagate_pieces<PL>(p, s, rc, t, b, q, cnt, lane);
+    return;
+  }
   if (cnt <= q) return;
   const uint32_t mine = (cnt - q + P - 1) / P;  // rows q, q+P, ... of the segment
   unsigned long long ev = 0, ltp = 0;
@@ -682,21 +772,7 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
     ltp += propagate_chunk<PL, BIN>(p, s, rc, t, total, lane, pos, bev);
     wave_sync();
   }
-  if (PL != NSX_PLAST_NONE) {
-#pragma unroll
-    for (uint32_t o = 32; o > 0; o >>= 1) ltp += __shfl_xor(ltp, o);
-  }
-  if (lane == 0) {
-    // parts of one block (or, in an epoch, several steps' items of one block) accumulate into
-    // that block's counter
-    if (P == 1 && !p.part_atomic) {
-      s.part[kPartEvents * p.nblk + b] += ev;
-      if (PL != NSX_PLAST_NONE) s.part[kPartLtp * p.nblk + b] += ltp;
-    } else {
-      atomicAdd(&s.part[kPartEvents * p.nblk + b], ev);
-      if (PL != NSX_PLAST_NONE) atomicAdd(&s.part[kPartLtp * p.nblk + b], ltp);
-    }
-  }
+  item_counters<PL>(p, s, b, ev, ltp, lane);
 }
 
 // Post-spike work item: part q of neuron block b (excitatory neurons only), over the E->E
@@ -836,7 +912,8 @@ __device__ inline void for_items(const StepParams& p, const DevPtrs& s, Fn&& fn)
       decode_item((uint32_t)mine, both, nprop, nltd, P, pr, bb, qq);
       c = s.bcnt[bb];
     }
-    unsigned long long act = __ballot(mine < nwork && c > qq);
+    // with a row-split threshold a part may hold pieces even when the block has fewer rows
+    unsigned long long act = __ballot(mine < nwork && c > (p.row_split ? 0u : qq));
     while (act) {
       const uint32_t l = (uint32_t)(__ffsll((long long)act) - 1);
       act &= act - 1ull;
@@ -885,16 +962,18 @@ __device__ inline void for_batches(const StepParams& p, const DevPtrs& s, PropFn
   }
 }
 
-// Seven wavefronts per SIMD (72 VGPRs) for the pair-STDP instance: fits without spills.
-template <int PL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
+// Seven wavefronts per SIMD (72 VGPRs) for the pair-STDP instance: fits without spills.  NT threads
+// per workgroup (64 .. 512; chosen by the autotuner): work items are per wavefront, the size only
+// changes how the wavefronts are grouped onto CUs.
+template <int PL, uint32_t NT, bool PIECES>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(7, 8)))
 void k_synapses(StepParams p, DevPtrs s, int32_t off) {
-  __shared__ RowChunk sh[kBlock / 64];
+  __shared__ RowChunk sh[NT / 64];
   const int32_t t = *s.tbase + off;
   const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  for_items<PL, kBlock>(p, s, [&](bool prop, uint32_t b, uint32_t q, uint32_t cnt) {
+  for_items<PL, NT>(p, s, [&](bool prop, uint32_t b, uint32_t q, uint32_t cnt) {
     if (prop)
-      propagate_item<PL, false>(p, s, sh[wid], t, b, q, cnt, lane, nullptr, nullptr);
+      propagate_item<PL, false, PIECES>(p, s, sh[wid], t, b, q, cnt, lane, nullptr, nullptr);
     else if (PL != NSX_PLAST_NONE)
       depress_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
   });
@@ -1083,7 +1162,7 @@ __global__ __launch_bounds__(kBlock) void k_epoch_update(StepParams p, DevPtrs s
 
 // The synapse pass of an epoch: propagation items (step k, block b, part q) of every step,
 // handed to the wavefronts as for_items does for one step.
-template <int PL>
+template <int PL, bool PIECES>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
 void k_synapses_epoch(StepParams p, DevPtrs s, EpochSeg ep, int32_t off, uint32_t len) {
   __shared__ RowChunk sh[kBlock / 64];
@@ -1104,7 +1183,7 @@ void k_synapses_epoch(StepParams p, DevPtrs s, EpochSeg ep, int32_t off, uint32_
       qq = rem - bb * P;
       c = ep.bcnt[(size_t)kk * p.nblk + bb];
     }
-    unsigned long long act = __ballot(mine < nwork && c > qq);
+    unsigned long long act = __ballot(mine < nwork && c > (p.row_split ? 0u : qq));
     while (act) {
       const uint32_t l = (uint32_t)(__ffsll((long long)act) - 1);
       act &= act - 1ull;
@@ -1116,8 +1195,8 @@ void k_synapses_epoch(StepParams p, DevPtrs s, EpochSeg ep, int32_t off, uint32_
       const uint32_t cnt = __builtin_amdgcn_readfirstlane(__shfl(c, (int)l));
       DevPtrs sk = s;
       sk.spikes = ep.spikes + ((size_t)kk * p.nblk) * kBlock;
-      propagate_item<PL, false>(p, sk, sh[wid], t0 + (int32_t)kk, b, q, cnt, lane, nullptr,
-                                nullptr);
+      propagate_item<PL, false, PIECES>(p, sk, sh[wid], t0 + (int32_t)kk, b, q, cnt, lane,
+                                        nullptr, nullptr);
     }
   }
 }
@@ -1514,9 +1593,35 @@ UpdateFn pick_update(int model, int drive, int pl, bool four) {
 // idle and the cap bounds their cost.
 uint32_t syn_grid(const nsx_ctx* ctx) {
   const bool plast = ctx->cfg.plasticity != NSX_PLAST_NONE;
-  const uint64_t waves = (uint64_t)ctx->p.nblk * (plast ? 2 : 1) * ctx->p.prop_split;
-  const uint64_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
-  return (uint32_t)std::min<uint64_t>(blocks, 16384);
+  const uint64_t waves = std::min<uint64_t>(
+      (uint64_t)ctx->p.nblk * (plast ? 2 : 1) * ctx->p.prop_split, 65536);
+  const uint32_t wpb = ctx->syn_block / 64;
+  return (uint32_t)std::max<uint64_t>((waves + wpb - 1) / wpb, 1);
+}
+
+typedef void (*SynFn)(StepParams, DevPtrs, int32_t);
+
+template <int PL, bool PIECES>
+static SynFn syn_kernel_nt(uint32_t nt) {
+  switch (nt) {
+    case 64: return k_synapses<PL, 64, PIECES>;
+    case 128: return k_synapses<PL, 128, PIECES>;
+    case 512: return k_synapses<PL, 512, PIECES>;
+    default: return k_synapses<PL, 256, PIECES>;
+  }
+}
+
+template <bool PIECES>
+static SynFn syn_kernel_pl(int pl, uint32_t nt) {
+  switch (pl) {
+    case NSX_PLAST_PAIR: return syn_kernel_nt<NSX_PLAST_PAIR, PIECES>(nt);
+    case NSX_PLAST_TRIPLET: return syn_kernel_nt<NSX_PLAST_TRIPLET, PIECES>(nt);
+    default: return syn_kernel_nt<NSX_PLAST_NONE, PIECES>(nt);
+  }
+}
+
+static SynFn syn_kernel(int pl, uint32_t nt, bool pieces) {
+  return pieces ? syn_kernel_pl<true>(pl, nt) : syn_kernel_pl<false>(pl, nt);
 }
 
 // Enqueue the launches of one step with offset `off` from *tbase.
@@ -1575,15 +1680,8 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
     if (e != hipSuccess) return e;
   } else {
     if (evs) (void)hipEventRecord(evs[2], ctx->stream);
-    if (pl == NSX_PLAST_PAIR)
-      hipLaunchKernelGGL(k_synapses<NSX_PLAST_PAIR>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
-                         ctx->d, off);
-    else if (pl == NSX_PLAST_TRIPLET)
-      hipLaunchKernelGGL(k_synapses<NSX_PLAST_TRIPLET>, dim3(gs), dim3(kBlock), 0, ctx->stream,
-                         p, ctx->d, off);
-    else
-      hipLaunchKernelGGL(k_synapses<NSX_PLAST_NONE>, dim3(gs), dim3(kBlock), 0, ctx->stream, p,
-                         ctx->d, off);
+    hipLaunchKernelGGL(syn_kernel(pl, ctx->syn_block, p.row_split != 0), dim3(gs),
+                       dim3(ctx->syn_block), 0, ctx->stream, p, ctx->d, off);
     if (evs) (void)hipEventRecord(evs[3], ctx->stream);
   }
   if (evs) (void)hipEventRecord(evs[4], ctx->stream);
@@ -1591,6 +1689,7 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
 }
 
 typedef void (*EpochFn)(StepParams, DevPtrs, EpochSeg, int32_t, uint32_t);
+typedef void (*EpochSynFn)(StepParams, DevPtrs, EpochSeg, int32_t, uint32_t);
 
 template <int MODEL>
 static EpochFn epoch_pick_drive(int drive) {
@@ -1659,8 +1758,10 @@ hipError_t enqueue_epoch(nsx_ctx* ctx, int32_t off, uint32_t len, hipEvent_t* ev
   }
   const uint64_t waves = (uint64_t)len * p.nblk * p.prop_split;
   const uint32_t gs = (uint32_t)std::min<uint64_t>((waves + kBlock / 64 - 1) / (kBlock / 64), 16384);
-  hipLaunchKernelGGL(k_synapses_epoch<NSX_PLAST_NONE>, dim3(std::max(gs, 1u)), dim3(kBlock), 0,
-                     ctx->stream, p, ctx->d, ctx->ep, off, len);
+  EpochSynFn syn = p.row_split ? k_synapses_epoch<NSX_PLAST_NONE, true>
+                               : k_synapses_epoch<NSX_PLAST_NONE, false>;
+  hipLaunchKernelGGL(syn, dim3(std::max(gs, 1u)), dim3(kBlock), 0, ctx->stream, p, ctx->d, ctx->ep,
+                     off, len);
   if (evs) {
     (void)hipEventRecord(evs[3], ctx->stream);
     (void)hipEventRecord(evs[4], ctx->stream);

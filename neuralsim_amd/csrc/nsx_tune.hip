@@ -2,10 +2,11 @@
 //
 // Chooses, by measurement on the context's own network and state, the launch parameters the
 // step's cost depends on beyond the network itself: the propagation split (wavefronts that
-// share one neuron block's spike segment -- the row-split of busy blocks with high-degree
-// rows), the graph window (steps per captured hipGraph launch) and the delivery (ring atomics
-// or binned, with its tile size).  The neuron block size is
-// fixed at 256 by the spike-segment layout.  Every trial times the same steps from a
+// share one neuron block's spike segment), the row-split threshold that cuts high-degree rows
+// into pieces those wavefronts share, the synapse pass's workgroup size, the point models'
+// neurons per update thread, the graph window (steps per captured hipGraph launch) and the
+// delivery (ring atomics or binned, with its tile size).  The neuron block of the spike
+// segments stays 256 neurons.  Every trial times the same steps from a
 // device-side snapshot of the mutable state (after its warm-up / capture the state is restored)
 // and the state is restored afterwards, so tuning does not change the simulation.
 #include <hip/hip_runtime.h>
@@ -107,9 +108,9 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     return set_error(ctx, NSX_E_STATE, "surrogate parameters not set");
   nsx_status s0 = bin_materialize(ctx);  // the snapshot's ring then holds all pending input
   if (s0) return s0;
+  if ((s0 = epoch_prepare(ctx))) return s0;
   const std::vector<Buf> bufs = mutable_buffers(ctx);
-  size_t total = 0;
-  for (const Buf& b : bufs) total += (b.bytes + 255) & ~(size_t)255;
+  const size_t total = snapshot_bytes(bufs);
   char* snap = nullptr;
   nsx_status s = dev_alloc(ctx, (void**)&snap, total, false);
   if (s) return s;
@@ -119,25 +120,35 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   const bool prof = ctx->profiling;
   const uint32_t split0 = ctx->p.prop_split, graph0 = ctx->graph_steps;
   const uint32_t deliv0 = ctx->delivery, te0 = ctx->bin_te_req;
+  const uint32_t rs0 = ctx->p.row_split, nt0 = ctx->syn_block, width0 = ctx->update_width;
   const uint64_t cap0 = ctx->bin_cap_req;
   const uint32_t stats0 = ctx->p.stats_on;  // trials neither count nor fold statistics
+  const bool point = ctx->cfg.model == NSX_MODEL_LIF || ctx->cfg.model == NSX_MODEL_IZHIKEVICH;
+  const bool per_step = epoch_len(ctx) == 1;  // an epoch launch has its own update and block
   ctx->profiling = false;
   ctx->p.stats_on = 0;
   destroy_graph(ctx);
   nsx_tune_report r{};
   double best = 1e300;
-  uint32_t best_split = split0, best_graph = graph0 ? graph0 : 50;
-  uint32_t best_te = deliv0 == NSX_DELIVERY_BINNED ? ctx->bin_te : 0u;  // 0: atomic
-  // te = 0: atomic delivery, else binned with that tile size
-  auto trial = [&](uint32_t split, uint32_t graph, uint32_t te) -> nsx_status {
-    nsx_status st = bin_configure(ctx, te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0, te);
-    if (!st) st = nsx_set_prop_split(ctx, split);
-    if (!st) st = nsx_set_graph_steps(ctx, graph);
+  struct Shape {
+    uint32_t split, graph, te, rs, nt, width;  // te 0: atomic delivery, else binned tile size
+  };
+  Shape cur{split0, graph0 ? graph0 : 50, deliv0 == NSX_DELIVERY_BINNED ? ctx->bin_te : 0u, rs0,
+            nt0, width0};
+  Shape bs = cur;
+  auto trial = [&](const Shape& sh) -> nsx_status {
+    nsx_status st = bin_configure(ctx, sh.te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0,
+                                  sh.te);
+    if (!st) st = nsx_set_prop_split(ctx, sh.split);
+    if (!st) st = nsx_set_row_split(ctx, sh.rs);
+    if (!st) st = nsx_set_syn_block(ctx, sh.nt);
+    if (!st) st = nsx_set_update_width(ctx, sh.width);
+    if (!st) st = nsx_set_graph_steps(ctx, sh.graph);
     if (st) return st;
     // the same number of steps in every trial: a multiple of every window tried (200)
     const uint64_t steps = (trial_steps + 199) / 200 * 200;
     nsx_step_stats ss{};
-    if ((st = nsx_step(ctx, graph ? graph : 8, &ss))) return st;  // capture / warm up
+    if ((st = nsx_step(ctx, sh.graph ? sh.graph : 8, &ss))) return st;  // capture / warm up
     // every trial times the same steps: back to the snapshot (the captured graph stays valid:
     // the buffers keep their addresses and the step base is part of the snapshot)
     hipError_t e0 = copy_all(ctx, bufs, snap, false);
@@ -147,18 +158,19 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     if ((st = bin_clear(ctx))) return st;
     if ((st = nsx_step(ctx, steps, &ss))) return st;
     const double us = ss.wall_ms * 1000.0 / (double)steps;
-    if (r.n_trials < 32) {
-      r.trial_split[r.n_trials] = split;
-      r.trial_graph[r.n_trials] = graph;
-      r.trial_tile[r.n_trials] = te;
-      r.us_per_step[r.n_trials] = us;
-      ++r.n_trials;
+    if (r.n_trials < 48) {
+      const uint32_t k = r.n_trials++;
+      r.trial_split[k] = sh.split;
+      r.trial_graph[k] = sh.graph;
+      r.trial_tile[k] = sh.te;
+      r.trial_row_split[k] = sh.rs;
+      r.trial_syn_block[k] = sh.nt;
+      r.trial_width[k] = sh.width;
+      r.us_per_step[k] = us;
     }
     if (us < best) {
       best = us;
-      best_split = split;
-      best_graph = graph;
-      best_te = te;
+      bs = sh;
     }
     hipError_t e = copy_all(ctx, bufs, snap, false);
     if (e != hipSuccess) return hip_error(ctx, e, "autotune restore");
@@ -166,22 +178,70 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     ctx->tbase_host = tb0;
     return bin_clear(ctx);  // the trial's events in flight are not part of the snapshot
   };
-  // coordinate search: the split at a 50-step window, then the window at the best split
   static const uint32_t kSplits[] = {1, 2, 4, 8, 16};
   static const uint32_t kGraphs[] = {10, 25, 100, 200};
-  const uint32_t te_start = best_te;
-  for (uint32_t sp : kSplits)
-    if ((s = trial(sp, 50, te_start))) break;
-  const uint32_t sp_best = best_split;
-  if (!s)
-    for (uint32_t g : kGraphs)
-      if ((s = trial(sp_best, g, te_start))) break;
-  // the other delivery: atomic, and binned where (D - 1) * tiles fits
-  const uint32_t sp2 = best_split, g2 = best_graph;
-  for (uint32_t te : {0u, 8192u, 16384u}) {
-    if (s || te == te_start) continue;
-    if (te && !bin_supported(ctx, te)) continue;
-    s = trial(sp2, g2, te);
+  // 1. the split at a 50-step window and the current delivery
+  for (uint32_t sp : kSplits) {
+    Shape sh = cur;
+    sh.split = sp;
+    sh.graph = 50;
+    if (!s) s = trial(sh);
+  }
+  // 2. atomic delivery: the row-split threshold for high-degree rows, the workgroup size
+  if (!s && bs.te == 0) {
+    const Shape base = bs;
+    for (uint32_t rs : {256u, 128u, 64u}) {
+      Shape sh = base;
+      sh.rs = rs;
+      if (!s && rs != base.rs) s = trial(sh);
+    }
+    if (per_step) {
+      const Shape base2 = bs;
+      for (uint32_t nt : {128u, 256u, 512u}) {
+        Shape sh = base2;
+        sh.nt = nt;
+        if (!s && nt != base2.nt) s = trial(sh);
+      }
+    }
+  }
+  // 3. the point models' neurons per update thread
+  if (!s && point && per_step) {
+    const Shape base = bs;
+    for (uint32_t w : {1u, 4u}) {
+      Shape sh = base;
+      sh.width = w;
+      if (!s) s = trial(sh);
+    }
+  }
+  // 4. the graph window
+  if (!s) {
+    const Shape base = bs;
+    for (uint32_t g : kGraphs) {
+      Shape sh = base;
+      sh.graph = g;
+      if (!s) s = trial(sh);
+    }
+  }
+  // 5. the other deliveries, and the split again under a delivery that wins
+  const uint32_t te_start = cur.te;
+  if (!s) {
+    const Shape base = bs;
+    for (uint32_t te : {0u, 8192u, 16384u}) {
+      if (s || te == te_start) continue;
+      if (te && !bin_supported(ctx, te)) continue;
+      Shape sh = base;
+      sh.te = te;
+      if (te) sh.rs = 0;  // binned passes walk whole rows
+      s = trial(sh);
+    }
+  }
+  if (!s && bs.te != te_start) {
+    const Shape base = bs;
+    for (uint32_t sp : kSplits) {
+      Shape sh = base;
+      sh.split = sp;
+      if (!s && sp != base.split) s = trial(sh);
+    }
   }
   dev_free(ctx, snap);
   ctx->profiling = prof;
@@ -190,18 +250,25 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   if (s) {
     (void)bin_configure(ctx, deliv0, cap0, te0);
     (void)nsx_set_prop_split(ctx, split0);
+    (void)nsx_set_row_split(ctx, rs0);
+    (void)nsx_set_syn_block(ctx, nt0);
+    (void)nsx_set_update_width(ctx, width0);
     (void)nsx_set_graph_steps(ctx, graph0);
     return s;
   }
-  if ((s = bin_configure(ctx, best_te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0,
-                         best_te)))
+  if ((s = bin_configure(ctx, bs.te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0, bs.te)))
     return s;
-  if ((s = nsx_set_prop_split(ctx, best_split))) return s;
-  if ((s = nsx_set_graph_steps(ctx, best_graph))) return s;
-  r.prop_split = best_split;
-  r.graph_steps = best_graph;
-  r.delivery = best_te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC;
-  r.tile_entries = best_te;
+  if ((s = nsx_set_prop_split(ctx, bs.split)) || (s = nsx_set_row_split(ctx, bs.rs)) ||
+      (s = nsx_set_syn_block(ctx, bs.nt)) || (s = nsx_set_update_width(ctx, bs.width)) ||
+      (s = nsx_set_graph_steps(ctx, bs.graph)))
+    return s;
+  r.prop_split = bs.split;
+  r.graph_steps = bs.graph;
+  r.delivery = bs.te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC;
+  r.tile_entries = bs.te;
+  r.row_split = bs.rs;
+  r.syn_block = bs.nt;
+  r.update_width = point ? bs.width : 0u;
   r.best_us_per_step = best;
   if (rep) *rep = r;
   return NSX_OK;

@@ -213,16 +213,34 @@ class SimulationEngine:
         return m.value
 
     def autotune(self, trial_steps=100):
-        """Measure and set the fastest propagation split, graph window and delivery (state
-        unchanged); returns (chosen split, chosen window, [(split, window, us per step, tile),
-        ...]) with tile 0 for atomic delivery, else the binned tile size; the chosen delivery is
-        then `delivery` / `last_tune.tile_entries`."""
+        """Measure and set the fastest launch shape (state unchanged); returns (chosen split,
+        chosen window, [(split, window, us per step, tile), ...]) with tile 0 for atomic
+        delivery, else the binned tile size.  The other choices (delivery, tile, row-split
+        threshold, synapse workgroup, update width) and their trials are in `last_tune` and
+        `tune_trials()`."""
         r = abi.NsxTuneReport()
         self._chk(self._lib.nsx_autotune(self._h, int(trial_steps), C.byref(r)), "nsx_autotune")
         self.last_tune = r
         trials = [(r.trial_split[k], r.trial_graph[k], r.us_per_step[k], r.trial_tile[k])
                   for k in range(r.n_trials)]
         return r.prop_split, r.graph_steps, trials
+
+    def tune_trials(self):
+        """Every trial of the last autotune as a dict."""
+        r = self.last_tune
+        return [{"split": r.trial_split[k], "graph": r.trial_graph[k], "tile": r.trial_tile[k],
+                 "row_split": r.trial_row_split[k], "syn_block": r.trial_syn_block[k],
+                 "width": r.trial_width[k], "us_per_step": r.us_per_step[k]}
+                for k in range(r.n_trials)]
+
+    def set_row_split(self, slots):
+        """Cut spiking rows longer than `slots` into pieces the block's wavefronts share
+        (atomic delivery; 0: whole rows)."""
+        self._chk(self._lib.nsx_set_row_split(self._h, slots), "nsx_set_row_split")
+
+    def set_syn_block(self, threads):
+        """Threads per workgroup of the atomic-delivery synapse pass (64 .. 512)."""
+        self._chk(self._lib.nsx_set_syn_block(self._h, threads), "nsx_set_syn_block")
 
     def snapshot_save(self):
         """Keep a device copy of everything a step changes (one per engine)."""

@@ -269,7 +269,8 @@ def bench_lif10k(ns, cpu=True, budget_s=8.0):
     return res
 
 
-def _timed(eng, steps, graph):
+def _timed(eng, steps, graph, epoch=128):
+    eng.set_epoch(epoch)
     eng.set_graph_steps(graph)
     eng.step(max(graph, 1) * 2)  # capture and warm up
     t0 = time.perf_counter()
@@ -277,19 +278,45 @@ def _timed(eng, steps, graph):
     return time.perf_counter() - t0, st
 
 
+def _kernel_us(eng, steps):
+    """Per-step device time of the update and synapse kernels over `steps` direct-launch steps
+    (an epoch's launch covers several steps)."""
+    eng.set_graph_steps(0)
+    eng.set_profiling(True)
+    eng.step(steps)
+    eng.set_profiling(False)
+    ms_u, _, _ = eng.kernel_time(0)
+    ms_s, _, _ = eng.kernel_time(1)
+    return ms_u * 1e3 / steps, ms_s * 1e3 / steps
+
+
 HH_LEGS = ("surrogate", "plain_dt0.01", "plain_dt0.05_x5", "accuracy")
 
 
-def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS, cpu=False, budget_s=8.0):
+def bench_hh(ns, n=200_000, sim_ms=100.0, graph=100, legs=HH_LEGS, cpu=False, budget_s=8.0):
     """HH network of 200k neurons: the learned surrogate (dt 0.05 ms) against the plain
-    integrator at dt 0.01 ms and against plain five-sub-step fusion at dt 0.05 ms; per-kernel
-    MFMA rate of the surrogate update; spike displacement of the surrogate against the plain
-    FP32 integrator on an uncoupled population under the same drive.  `legs` selects a subset
-    (one engine per process keeps a kernel trace to that leg)."""
+    integrator at dt 0.01 ms and against plain five-sub-step fusion at dt 0.05 ms, each run
+    with one launch pair per step ("per_step") and in epochs of d_min steps ("epoch": 1 ms of
+    simulated time per update launch, the shortest delay); per-kernel MFMA rate of the surrogate
+    update; spike displacement of the surrogate against the plain FP32 integrator on an
+    uncoupled population under the same drive.  `legs` selects a subset (one engine per process
+    keeps a kernel trace to that leg)."""
     from neuralsim_amd import configs, surrogate
     from neuralsim_amd.metrics import spike_displacement
     res = {}
     params = norm = None
+
+    def both(eng, name, dt, steps):
+        out = {}
+        for mode, ep in (("per_step", 1), ("epoch", 128)):
+            wall, st = _timed(eng, steps, graph, ep)
+            uk, sk = _kernel_us(eng, min(steps, 2000))
+            out[mode] = {"epoch_steps": eng.epoch, "ms_per_sim_s": wall / sim_ms * 1e6,
+                         "us_per_step": wall / steps * 1e6, "update_us_per_step": uk,
+                         "synapse_us_per_step": sk,
+                         "rate_hz": st.spikes / n / (sim_ms / 1e3)}
+        return out
+
     if "surrogate" in legs:
         spec = configs.hh_surrogate_network(n=n)
         with ns.SimulationEngine.from_spec(spec) as eng:
@@ -298,19 +325,11 @@ def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS, cpu=False, bud
             res["fit"] = {"samples": fit["samples"], "generate_s": fit["generate_s"],
                           "train_s": fit["train_s"], "steps": len(fit["loss"]),
                           "final_loss": float(fit["loss"][-200:].mean())}
-            steps = int(round(sim_ms / 0.05))
-            wall, st = _timed(eng, steps, graph)
-            res["surrogate"] = {"dt_ms": 0.05, "ms_per_sim_s": wall / sim_ms * 1e6,
-                                "us_per_step": wall / steps * 1e6,
-                                "rate_hz": st.spikes / n / (sim_ms / 1e3),
-                                "fallback_neuron_steps": eng.surrogate_fallbacks()}
-            eng.set_profiling(True)
-            eng.step(200)
-            ms, launches, _ = eng.kernel_time(0)
-            eng.set_profiling(False)
-            us = ms / max(launches, 1) * 1e3
+            res["surrogate"] = {"dt_ms": 0.05, **both(eng, "surrogate", 0.05,
+                                                      int(round(sim_ms / 0.05)))}
+            res["surrogate"]["fallback_neuron_steps"] = eng.surrogate_fallbacks()
+            us = res["surrogate"]["epoch"]["update_us_per_step"]
             flop = 2.0 * (5 * 64 + 64 * 64 + 2 * 64 * 4) * n  # W3 runs as two bf16 words
-            res["surrogate"]["update_kernel_us"] = us
             res["surrogate"]["mlp_tflops"] = flop / (us * 1e-6) / 1e12
             res["surrogate"]["mlp_frac_of_bf16_peak"] = flop / (us * 1e-6) / BF16_PEAK
     for name, dt, sub in [("plain_dt0.01", 0.01, 1), ("plain_dt0.05_x5", 0.05, 5)]:
@@ -318,11 +337,8 @@ def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS, cpu=False, bud
             continue
         with ns.SimulationEngine.from_spec(configs.hh_network(n=n, dt_ms=dt,
                                                               hh_substeps=sub)) as eng:
-            steps = int(round(sim_ms / dt))
-            wall, st = _timed(eng, steps, graph)
-            res[name] = {"dt_ms": dt, "substeps": sub, "ms_per_sim_s": wall / sim_ms * 1e6,
-                         "us_per_step": wall / steps * 1e6,
-                         "rate_hz": st.spikes / n / (sim_ms / 1e3)}
+            res[name] = {"dt_ms": dt, "substeps": sub,
+                         **both(eng, name, dt, int(round(sim_ms / dt)))}
             if cpu and name == "plain_dt0.01":
                 log("cpu baseline: HH 200k/20M port")
                 csr = eng.read_csr()
@@ -335,10 +351,17 @@ def bench_hh(ns, n=200_000, sim_ms=100.0, graph=50, legs=HH_LEGS, cpu=False, bud
                     f"OpenMP neuron phase); value = 0.01 ms / seconds per step",
                     {"ms_per_step": s_step * 1e3, "ms_per_sim_s": s_step * 1e5 * 1e3})
     if "surrogate" in res:
-        s_ms = res["surrogate"]["ms_per_sim_s"]
+        sur = res["surrogate"]
         for name in ("plain_dt0.01", "plain_dt0.05_x5"):
             if name in res:
-                res[f"speedup_vs_{name}"] = res[name]["ms_per_sim_s"] / s_ms
+                # same launch structure on both sides, and each side per step
+                res[f"speedup_vs_{name}"] = (res[name]["epoch"]["ms_per_sim_s"]
+                                             / sur["epoch"]["ms_per_sim_s"])
+                res[f"speedup_vs_{name}_per_step"] = (res[name]["per_step"]["ms_per_sim_s"]
+                                                      / sur["per_step"]["ms_per_sim_s"])
+        if "plain_dt0.01" in res:
+            res["speedup_epoch_vs_plain_dt0.01_per_step"] = (
+                res["plain_dt0.01"]["per_step"]["ms_per_sim_s"] / sur["epoch"]["ms_per_sim_s"])
     if "accuracy" in legs and params is not None:
         # 2000 uncoupled neurons, 500 ms, same OU drive
         kw = dict(k_in=1, w_exc=(0.0, 0.0), w_inh=(0.0, 0.0), record_raster=True,
@@ -376,11 +399,24 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
         row_split, syn_block, width = int(r.row_split), int(r.syn_block), int(r.update_width)
         trials = [{k: (round(v, 1) if k == "us_per_step" else int(v)) for k, v in t.items()}
                   for t in pop.tune_trials()]
+        pop.snapshot_save()
+        pop.step(graph)  # capture the tuned window's graph outside the timed region
+        pop.snapshot_restore()
         t0 = time.perf_counter()
         st = pop.step(steps - warm)
         wall = time.perf_counter() - t0
         spikes, _ = pop.net_stats()
         deliv, tile = pop.delivery, int(pop.last_tune.tile_entries)
+        # the timed steps again from the snapshot as direct launches bracketed by per-kernel
+        # hipEvents (the replay repeats them bit for bit)
+        pop.snapshot_restore()
+        pop.set_profiling(True)
+        pst = pop.step(steps - warm)
+        pop.set_profiling(False)
+        assert pst.events == st.events and pst.spikes == st.spikes
+        kt = {name: pop.kernel_time(k)[0] * 1e3 / (steps - warm)
+              for k, name in ((0, "update"), (2, "bin_counting"), (1, "synapses"),
+                              (3, "bin_arrival"))}
     n = np.array([b.n_neurons for b in nets], np.float64)
     m = np.array([b.n_neurons * b.k_in for b in nets], np.float64)
     rate = spikes / n / (steps * base.dt_ms / 1e3)
@@ -392,6 +428,7 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
                      "row_split": row_split, "syn_block": syn_block, "update_width": width,
                      "trials": trials},
            "timed_steps": steps - warm,
+           "kernel_us_per_step": kt,
            "ms_per_step": wall * 1e3 / (steps - warm),
            "x_realtime": (steps - warm) * base.dt_ms / 1e3 / wall,
            "syn_events_per_s": st.events / wall, "spikes": int(spikes.sum()),

@@ -407,16 +407,16 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
         wall = time.perf_counter() - t0
         spikes, _ = pop.net_stats()
         deliv, tile = pop.delivery, int(pop.last_tune.tile_entries)
-        # the timed steps again from the snapshot as direct launches bracketed by per-kernel
-        # hipEvents (the replay repeats them bit for bit)
+        # the timed steps again from the snapshot as direct launches bracketed by hipEvents
+        # (the replay repeats them bit for bit).  Only the sum over a step's kernels is kept:
+        # between adjacent kernels the event stamps shift time from one to the other (a kernel
+        # trace of the same launches: population_256_izh_kernels.md)
         pop.snapshot_restore()
         pop.set_profiling(True)
         pst = pop.step(steps - warm)
         pop.set_profiling(False)
         assert pst.events == st.events and pst.spikes == st.spikes
-        kt = {name: pop.kernel_time(k)[0] * 1e3 / (steps - warm)
-              for k, name in ((0, "update"), (2, "bin_counting"), (1, "synapses"),
-                              (3, "bin_arrival"))}
+        kt = sum(pop.kernel_time(k)[0] for k in range(4)) * 1e3 / (steps - warm)
     n = np.array([b.n_neurons for b in nets], np.float64)
     m = np.array([b.n_neurons * b.k_in for b in nets], np.float64)
     rate = spikes / n / (steps * base.dt_ms / 1e3)
@@ -428,7 +428,7 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
                      "row_split": row_split, "syn_block": syn_block, "update_width": width,
                      "trials": trials},
            "timed_steps": steps - warm,
-           "kernel_us_per_step": kt,
+           "replay_kernel_us_per_step": kt,
            "ms_per_step": wall * 1e3 / (steps - warm),
            "x_realtime": (steps - warm) * base.dt_ms / 1e3 / wall,
            "syn_events_per_s": st.events / wall, "spikes": int(spikes.sum()),
@@ -438,6 +438,23 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
     if cpu:
         res["cpu_baseline"] = cpu_population(base, nets, m, budget_s)
     return res
+
+
+def bench_population_direct(ns, warm=200, steps=200, split=8, tile=16384):
+    """The population with the shape the autotuner picks on this machine (propagation split 8,
+    binned delivery with 16384-entry tiles), no tuning and no graphs: every step is direct
+    launches, so a kernel trace sees each one.  `warm` steps, then `steps` timed."""
+    from neuralsim_amd import configs
+    with ns.BatchedPopulation(configs.population_base(), configs.population_nets(256)) as pop:
+        pop.set_graph_steps(0)
+        pop.set_prop_split(split)
+        pop.set_delivery(1, 0, tile)
+        pop.step(warm)
+        t0 = time.perf_counter()
+        st = pop.step(steps)
+        wall = time.perf_counter() - t0
+    return {"split": split, "tile_entries": tile, "warm_steps": warm, "steps": steps,
+            "ms_per_step": wall * 1e3 / steps, "events_per_step": st.events / steps}
 
 
 def cpu_population(base, nets, m, budget_s, stride=16, steps=20):
@@ -554,7 +571,7 @@ def probe_atomics(reps=20):
 
 
 LEGS = ("headline", "lif10k", "hh_surrogate", "hh_plain", "hh_plain_x5", "population",
-        "large", "large_active")
+        "population_direct", "large", "large_active")
 
 
 def main():
@@ -618,6 +635,8 @@ def main():
             res.update(bench_hh(ns, legs=(leg,)))
         elif args.leg == "population":
             res.update(bench_population(ns))
+        elif args.leg == "population_direct":
+            res.update(bench_population_direct(ns))
         else:
             scale, steps = (1.0, 300) if args.leg == "large" else (60.0, 100)
             res.update(bench_large(ns, steps=steps, drive_scale=scale))

@@ -406,10 +406,12 @@ typedef struct nsx_batch_net {
   float w_exc_lo, w_exc_hi, w_inh_lo, w_inh_hi;
 } nsx_batch_net;
 
-/* base supplies the model (LIF, Izhikevich or HH), dt, drive and neuron parameters shared by
- * all networks (its n_neurons, n_exc and seed are ignored); plasticity must be NSX_PLAST_NONE.
- * Each network keeps its own E/I split: its neuron types and, in two-channel models, the ring
- * channel its inhibitory sources feed. */
+/* base supplies the model (LIF, Izhikevich or HH), dt, drive, neuron parameters and plasticity
+ * rule shared by all networks (its n_neurons, n_exc and seed are ignored).  Each network keeps
+ * its own E/I split: its neuron types, in two-channel models the ring channel its inhibitory
+ * sources feed, and under pair / triplet STDP its E->E synapses (the learning of each network is
+ * that of its separate context: the Lamarckian evaluation of population.rs:126-138 as one
+ * batch). */
 nsx_status nsx_batch_create(const nsx_config* base, const nsx_batch_net* nets, uint32_t n_nets,
                             nsx_ctx** out);
 /* offsets[n_nets + 1]: first global neuron of each network; offsets[n_nets] = total neurons. */

@@ -549,6 +549,14 @@ static nsx_status put_post(nsx_ctx* ctx, const float* v, const int32_t* st, cons
   return NSX_OK;
 }
 
+// Whether neuron j is excitatory (in a batched population: local index < its network's n_exc).
+static bool host_exc(const nsx_ctx* ctx, uint32_t j) {
+  if (!ctx->p.batch) return j < ctx->cfg.n_exc;
+  for (const BatchNet& b : ctx->nets_h)
+    if (j >= b.offset && j - b.offset < b.n) return j - b.offset < b.n_exc;
+  return false;
+}
+
 // Post records in the reference's view: the bumps of the last step's spikes, which the device
 // applies at the start of the next step, are applied here with the same float operations.
 static nsx_status get_post(nsx_ctx* ctx, float* v, int32_t* st) {
@@ -565,7 +573,7 @@ static nsx_status get_post(nsx_ctx* ctx, float* v, int32_t* st) {
     for (size_t b = 0; b < nb; ++b)
       for (uint32_t k = 0; k < cnt[b]; ++k) {
         const uint32_t j = seg[b * kBlock + k];
-        if (j >= ctx->cfg.n_exc) continue;
+        if (!host_exc(ctx, j)) continue;
         const int32_t dk = t - h[j].s;
         const float q = (dk >= 0 && dk < ctx->p.mtab_n) ? ctx->mtab_h[(size_t)dk] : 0.0f;
         float x = h[j].v * q;

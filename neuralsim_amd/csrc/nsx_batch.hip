@@ -90,8 +90,6 @@ nsx_status nsx_batch_create(const nsx_config* base, const nsx_batch_net* nets, u
   if (!base || !nets || n_nets < 1) return set_error(nullptr, NSX_E_INVALID, "no networks");
   if (base->struct_size != sizeof(nsx_config))
     return set_error(nullptr, NSX_E_INVALID, "nsx_config.struct_size mismatch");
-  if (base->plasticity != NSX_PLAST_NONE)
-    return set_error(nullptr, NSX_E_INVALID, "a batched population runs without plasticity");
   if (base->model == NSX_MODEL_HH_SURROGATE)
     return set_error(nullptr, NSX_E_INVALID, "a batched population runs LIF, Izhikevich or HH");
   std::vector<BatchNet> tab(n_nets);

@@ -197,23 +197,37 @@ __global__ void k_check_csr(const uint64_t* row_ptr, const uint32_t* col, const 
   }
 }
 
+// Excitatory test of the transpose build: i < n_exc, and in a batched population (blk_net set,
+// n_exc = all neurons) local index < its network's n_exc.
+__device__ inline bool csc_exc(uint32_t i, uint32_t n_exc, const uint32_t* blk_net,
+                               const BatchNet* bnet) {
+  if (i >= n_exc) return false;
+  if (!blk_net) return true;
+  const BatchNet& b = bnet[blk_net[i / kBlock]];
+  return i - b.offset < b.n_exc;
+}
+
 // E->E transpose: count per excitatory target, then scatter (slot, source).
 __global__ void k_csc_count(const uint64_t* row_ptr, const uint32_t* col, uint32_t n_exc,
+                            const uint32_t* blk_net, const BatchNet* bnet,
                             unsigned long long* cnt) {
   for (uint32_t i = blockIdx.x; i < n_exc; i += gridDim.x) {  // a block per source row
+    if (!csc_exc(i, n_exc, blk_net, bnet)) continue;
     for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
       uint32_t j = col[k];
-      if (j < n_exc) atomicAdd(&cnt[j], 1ull);
+      if (csc_exc(j, n_exc, blk_net, bnet)) atomicAdd(&cnt[j], 1ull);
     }
   }
 }
 
 __global__ void k_csc_fill(const uint64_t* row_ptr, const uint32_t* col, uint32_t n_exc,
+                           const uint32_t* blk_net, const BatchNet* bnet,
                            unsigned long long* cursor, uint64_t* slot, uint32_t* srcs) {
   for (uint32_t i = blockIdx.x; i < n_exc; i += gridDim.x) {
+    if (!csc_exc(i, n_exc, blk_net, bnet)) continue;
     for (uint64_t k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x) {
       uint32_t j = col[k];
-      if (j < n_exc) {
+      if (csc_exc(j, n_exc, blk_net, bnet)) {
         unsigned long long p = atomicAdd(&cursor[j], 1ull);
         slot[p] = k;
         srcs[p] = i;
@@ -323,9 +337,9 @@ nsx_status build_csc_device(nsx_ctx* ctx) {
   if (ne == 0) return NSX_OK;
   unsigned long long* cnt = nullptr;
   if ((s = dev_alloc(ctx, (void**)&cnt, (size_t)ne * 8, true))) return s;
+  const uint32_t* bt = ctx->p.batch ? ctx->d.blk_net : nullptr;  // batch: per-network E/I
   hipLaunchKernelGGL(k_csc_count, dim3(blocks_for(ne, 1)), dim3(128), 0, ctx->stream,
-                     ctx->row_ptr, ctx->col,
-                     ne, cnt);
+                     ctx->row_ptr, ctx->col, ne, bt, ctx->d.bnet, cnt);
   NSX_HIP(ctx, hipGetLastError());
   // exclusive scan into csc_ptr[1..]
   size_t tmp_bytes = 0;
@@ -346,7 +360,8 @@ nsx_status build_csc_device(nsx_ctx* ctx) {
   NSX_HIP(ctx, hipMemcpyAsync(cnt, ctx->csc_ptr, (size_t)ne * 8, hipMemcpyDeviceToDevice,
                               ctx->stream));
   hipLaunchKernelGGL(k_csc_fill, dim3(blocks_for(ne, 1)), dim3(128), 0, ctx->stream,
-                     ctx->row_ptr, ctx->col, ne, cnt, ctx->csc_slot, ctx->csc_src);
+                     ctx->row_ptr, ctx->col, ne, bt, ctx->d.bnet, cnt, ctx->csc_slot,
+                     ctx->csc_src);
   NSX_HIP(ctx, hipGetLastError());
   unsigned int* bad = nullptr;
   if ((s = dev_alloc(ctx, (void**)&bad, 4, true))) return s;

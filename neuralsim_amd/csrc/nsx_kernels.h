@@ -63,6 +63,14 @@ __device__ inline NeuronId neuron_id(const StepParams& p, const DevPtrs& s, uint
   return NeuronId{seed, i - off, n, ne};
 }
 
+// Whether neuron i is excitatory: in [0, n_exc) of the network, or of its own network in a
+// batched population (the block table names it).  The single network's test costs no load.
+__device__ inline bool exc_neuron(const StepParams& p, const DevPtrs& s, uint32_t i) {
+  if (!p.batch) return i < p.n_exc;
+  const BatchNet& b = s.bnet[s.blk_net[i / kBlock]];
+  return i - b.offset < b.n_exc;
+}
+
 // Drive words of one step for a block's neurons.  One Philox draw covers four consecutive
 // neurons of a network (local indices 4g .. 4g+3; blocks start at a multiple of 256 of their
 // network): wavefront 0 draws the block's 64 groups into w[0 .. kBlock).  The caller synchronises
@@ -122,7 +130,7 @@ __device__ inline void drive_draw(const StepParams& p, const uint32_t* w, float&
 template <bool PAIR>
 __device__ inline void post_bump_prev(const StepParams& p, const DevPtrs& s, uint32_t i,
                                       int32_t last, int32_t t) {
-  if (PAIR && last == t - 1 && i < p.n_exc) {
+  if (PAIR && last == t - 1 && exc_neuron(p, s, i)) {
     PostRec& r = s.post[i];
     const float v = r.v;
     const int32_t st = r.s;
@@ -185,7 +193,7 @@ __device__ inline void spike_record(const StepParams& p, const DevPtrs& s, uint3
                                     int32_t t) {
   s.count[i] += 1u;
   s.last[i] = t;
-  if (PL == NSX_PLAST_PAIR && i < p.n_exc) {
+  if (PL == NSX_PLAST_PAIR && exc_neuron(p, s, i)) {
     s.pre[i] = trace_bump(s.pre[i], s.ptab, p.ptab_n, t);
     PostRec& r = s.post[i];
     r.last = t;
@@ -195,7 +203,7 @@ __device__ inline void spike_record(const StepParams& p, const DevPtrs& s, uint3
       r.npend = np + 1;
     }
   }
-  if (PL == NSX_PLAST_TRIPLET && i < p.n_exc) {
+  if (PL == NSX_PLAST_TRIPLET && exc_neuron(p, s, i)) {
     PostRec r = s.post[i];
     r.s = r.last;
     r.last = t;

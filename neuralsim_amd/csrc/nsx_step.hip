@@ -597,7 +597,8 @@ __device__ unsigned long long propagate_chunk(const StepParams& p, const DevPtrs
     TraceRec a[NS];
 #pragma unroll
     for (uint32_t q = 0; q < NS; ++q) {
-      pot[q] = PL != NSX_PLAST_NONE && has[q] && src[q] < p.n_exc && j[q] < p.n_exc;
+      pot[q] = PL != NSX_PLAST_NONE && has[q] && exc_neuron(p, s, src[q]) &&
+               exc_neuron(p, s, j[q]);
       pr[q] = PostRec{0.0f, kNeverStep, kNeverStep, 0u, {}};
       if (pot[q]) pr[q] = s.post[j[q]];  // 32 bytes: trace, last spike, pending spikes
       a[q] = TraceRec{};
@@ -716,7 +717,7 @@ __device__ void propagate_pieces(const StepParams& p, const DevPtrs& s, RowChunk
         pbeg = rb + (uint64_t)k * T;
         pdeg = min(T, rd - k * T);
         if (p.ch > 1) rc.chan[lane] = source_channel(p, s, pi);
-        if (PL == NSX_PLAST_PAIR && pi < p.n_exc) {
+        if (PL == NSX_PLAST_PAIR && exc_neuron(p, s, pi)) {
           const TraceRec a = s.pre[pi];
           rc.pre_v[lane] = a.v;
           rc.pre_vp[lane] = a.v_prev;
@@ -761,7 +762,7 @@ __device__ void propagate_item(const StepParams& p, const DevPtrs& s, RowChunk& 
       deg = (uint32_t)(s.row_ptr[i + 1] - beg);
     }
     if (p.ch > 1 && lane < nr) rc.chan[lane] = source_channel(p, s, i);
-    if (PL == NSX_PLAST_PAIR && lane < nr && i < p.n_exc) {
+    if (PL == NSX_PLAST_PAIR && lane < nr && exc_neuron(p, s, i)) {
       const TraceRec a = s.pre[i];
       rc.pre_v[lane] = a.v;
       rc.pre_vp[lane] = a.v_prev;
@@ -792,7 +793,7 @@ __device__ void depress_item(const StepParams& p, const DevPtrs& s, RowChunk& rc
     uint64_t beg = 0;
     if (lane < nr) {
       j = s.spikes[b * kBlock + q + (c0 + lane) * P];
-      if (j < p.n_exc && (PL != NSX_PLAST_PAIR || s.post[j].npend == kPostPend)) {
+      if (exc_neuron(p, s, j) && (PL != NSX_PLAST_PAIR || s.post[j].npend == kPostPend)) {
         beg = s.csc_ptr[j];
         deg = (uint32_t)(s.csc_ptr[j + 1] - beg);
       }
@@ -1276,7 +1277,7 @@ __device__ inline unsigned long long batch_rows(const StepParams& p, const DevPt
       beg = s.row_ptr[i];
       deg = (uint32_t)(s.row_ptr[i + 1] - beg);
       if (p.ch > 1) rc.chan[lane] = source_channel(p, s, i);
-      if (pre_rec && i < p.n_exc) {
+      if (pre_rec && exc_neuron(p, s, i)) {
         const TraceRec a = s.pre[i];
         rc.pre_v[lane] = a.v;
         rc.pre_vp[lane] = a.v_prev;

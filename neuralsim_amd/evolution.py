@@ -13,13 +13,13 @@ Evaluation, `Population.evaluate`:
   given its genome's weights (nsx_upload_weights) and stepped together for the lifetime; the
   fitness is RateHomeostasis of each network's count rate (population.evaluate's convention).
   Every network sees the same drive (the spec's seed keys both topology and drive), so
-  individuals differ only by their weights: common random numbers across the generation.  The
-  batched engine runs without plasticity, so Lamarckian extraction returns the weights as
-  evaluated.
-* "sequential": one SimulationEngine per genome with the base configuration (plasticity as
-  configured, e.g. pair STDP) -- the reference's per-genome build_engine + simulate_ms; with
-  `lamarckian` the learned weights are read back into the genome (nsx_read_weights), where the
-  reference re-runs a separate 500 ms simulation for them (population.rs:126-138).
+  individuals differ only by their weights: common random numbers across the generation.  Under
+  a plastic base configuration (pair or triplet STDP) each network learns as its own engine
+  would, and with `lamarckian` the learned weights are read back into the genomes
+  (nsx_read_weights of the batch, each network's slot range), where the reference re-runs a
+  separate 500 ms simulation per genome for them (population.rs:126-138).
+* "sequential": one SimulationEngine per genome with the base configuration -- the reference's
+  per-genome build_engine + simulate_ms, the same results as "batched" one engine at a time.
 
 Operators follow the reference's definitions with numpy's PCG64 in place of Rust's StdRng, so
 runs are reproducible here but not draw-for-draw equal to the reference's.  Weights carry
@@ -37,7 +37,7 @@ import numpy as np
 
 from . import abi, configs
 from .engine import BatchedPopulation, SimulationEngine
-from .population import RateHomeostasis
+from .population import RateHomeostasis, lifetime_steps
 from .stats import rust_display
 
 
@@ -222,8 +222,6 @@ class Population:
                  template=None):
         if mode not in ("batched", "sequential"):
             raise ValueError("mode must be 'batched' or 'sequential'")
-        if mode == "batched" and base.plasticity != abi.PLAST_NONE:
-            raise ValueError("the batched engine runs without plasticity: use mode='sequential'")
         self.genomes = list(genomes)
         self.generation = 0
         self.config = config
@@ -263,7 +261,7 @@ class Population:
     def evaluate(self, evaluator):
         """Fitness of every genome (population.rs:116-124); with `lamarckian`, the weights
         after the lifetime are written back into the genomes."""
-        steps = int(round(evaluator.lifetime_ms / self.base.dt_ms))
+        steps = lifetime_steps(evaluator.lifetime_ms, self.base.dt_ms)
         k = len(self.genomes)
         n = self.net.n_neurons
         if self.mode == "batched":

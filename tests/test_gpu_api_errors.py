@@ -46,9 +46,6 @@ def test_rejected_calls_change_nothing(nsx):
 def test_batch_and_surrogate_preconditions(nsx):
     base = configs.population_base()
     nets = configs.population_nets(3, seed=5, n_range=(300, 600))
-    plastic = base.copy()
-    plastic.plasticity = abi.PLAST_PAIR
-    assert _status(nsx, nsx.BatchedPopulation, plastic, nets) == "NSX_E_INVALID"
     sur = base.copy()
     sur.model = abi.MODEL_HH_SURROGATE
     assert _status(nsx, nsx.BatchedPopulation, sur, nets) == "NSX_E_INVALID"

@@ -1,7 +1,8 @@
 """The evolution loop on the device: a generation evaluated as one batched launch equals the
 same genomes run one network at a time (fitness, rates); weights uploaded into the batch land
 in the right networks; the reference's evolve test (population.rs:333-364: the best never
-decreases) on the device evaluation; Lamarckian extraction of STDP-learned weights."""
+decreases) on the device evaluation; Lamarckian extraction of STDP-learned weights, batched
+and sequential alike."""
 import numpy as np
 import pytest
 
@@ -63,11 +64,16 @@ def test_lamarckian_extracts_learned_weights(nsx):
     cfg = EvolutionConfig(seed=9, lamarckian=True)
     ev = RateHomeostasis(5.0, lifetime_ms=200.0)
     print("case evolution lamarckian", flush=True)
-    with pytest.raises(ValueError):
-        Population.random(2, cfg, base=base, net=_net(1000, 40))  # batched: no plasticity
-    pop = Population.random(2, cfg, base=base, net=_net(1000, 40), mode="sequential")
+    pop = Population.random(3, cfg, base=base, net=_net(1000, 40))  # batched, pair STDP
+    seq = Population([g.copy() for g in pop.genomes], cfg, pop.base, pop.net, mode="sequential",
+                     template=(pop.csr, pop.exc))
     before = [g.weights.copy() for g in pop.genomes]
-    pop.evaluate(ev)
+    fb = pop.evaluate(ev)
+    fs = seq.evaluate(ev)
+    # the batch learns what each genome's own engine learns, bit for bit
+    assert np.array_equal(fb, fs) and np.array_equal(pop.last_rates, seq.last_rates)
+    for g, h in zip(pop.genomes, seq.genomes):
+        assert np.array_equal(g.weights.view(np.uint32), h.weights.view(np.uint32))
     changed = [np.any(g.weights != b) for g, b in zip(pop.genomes, before)]
     assert all(changed)
     # only excitatory-to-excitatory synapses learn

@@ -424,7 +424,7 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
     res = {"networks": len(nets), "neurons": int(n.sum()), "synapses": int(m.sum()),
            "steps": steps, "build_s": build_s,
            "tuned": {"prop_split": split, "graph_steps": graph,
-                     "delivery": "binned" if deliv else "atomic", "tile_entries": tile,
+                     "delivery": ("atomic", "binned", "pull")[deliv], "tile_entries": tile,
                      "row_split": row_split, "syn_block": syn_block, "update_width": width,
                      "trials": trials},
            "timed_steps": steps - warm,
@@ -440,20 +440,23 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
     return res
 
 
-def bench_population_direct(ns, warm=200, steps=200, split=8, tile=16384):
-    """The population with the shape the autotuner picks on this machine (propagation split 8,
-    binned delivery with 16384-entry tiles), no tuning and no graphs: every step is direct
-    launches, so a kernel trace sees each one.  `warm` steps, then `steps` timed."""
-    from neuralsim_amd import configs
+def bench_population_direct(ns, delivery="binned", warm=200, steps=200, split=8, tile=16384):
+    """The population with a fixed shape (by default the one the autotuner picks on this
+    machine: binned delivery with 16384-entry tiles, propagation split 8; or pull or atomic
+    delivery), no tuning and no graphs: every step is direct launches, so a kernel trace sees
+    each one.  `warm` steps, then `steps` timed."""
+    from neuralsim_amd import abi, configs
+    mode = {"atomic": abi.DELIVERY_ATOMIC, "binned": abi.DELIVERY_BINNED,
+            "pull": abi.DELIVERY_PULL}[delivery]
     with ns.BatchedPopulation(configs.population_base(), configs.population_nets(256)) as pop:
         pop.set_graph_steps(0)
         pop.set_prop_split(split)
-        pop.set_delivery(1, 0, tile)
+        pop.set_delivery(mode, 0, tile if mode == abi.DELIVERY_BINNED else 0)
         pop.step(warm)
         t0 = time.perf_counter()
         st = pop.step(steps)
         wall = time.perf_counter() - t0
-    return {"split": split, "tile_entries": tile, "warm_steps": warm, "steps": steps,
+    return {"delivery": delivery, "split": split, "warm_steps": warm, "steps": steps,
             "ms_per_step": wall * 1e3 / steps, "events_per_step": st.events / steps}
 
 
@@ -597,6 +600,8 @@ def main():
     ap.add_argument("--no-lif10k", action="store_true", help="skip the 10k LIF network")
     ap.add_argument("--leg", choices=LEGS,
                     help="run only this workload (one engine per process, for kernel traces)")
+    ap.add_argument("--pop-delivery", choices=("binned", "pull", "atomic"), default="binned",
+                    help="delivery of the population_direct leg")
     ap.add_argument("--probe-atomics", action="store_true",
                     help="measure the device's random int64 / int32 atomic-add throughput only")
     ap.add_argument("--counters-window", action="store_true",
@@ -636,7 +641,7 @@ def main():
         elif args.leg == "population":
             res.update(bench_population(ns))
         elif args.leg == "population_direct":
-            res.update(bench_population_direct(ns))
+            res.update(bench_population_direct(ns, args.pop_delivery))
         else:
             scale, steps = (1.0, 300) if args.leg == "large" else (60.0, 100)
             res.update(bench_large(ns, steps=steps, drive_scale=scale))

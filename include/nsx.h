@@ -258,10 +258,20 @@ nsx_status nsx_get_epoch(nsx_ctx* ctx, uint32_t* steps);
  *   M < 2^32; a step with more events than cap_events (0: min(M, 2^24, 2^31 / (8 D)), at least
  *   min(M, 2^16)) falls back to atomics.  Costs D * cap_events * 8 bytes.  Chosen before a
  *   network is loaded, BINNED is sized at the next nsx_step; if that network does not meet the
- *   limits, the call fails with NSX_E_INVALID and the engine continues with ATOMIC. */
+ *   limits, the call fails with NSX_E_INVALID and the engine continues with ATOMIC.
+ * PULL (networks without plasticity): no synapse pass.  After the update of step t, one
+ *   workgroup per target tile (a range of at most 16384 / channels neurons of one network)
+ *   gathers the input of step t + 1 from the spikes of steps t + 2 - D .. t: for each such spike
+ *   it reads the slice of the source's row that targets the tile with the delay that arrives
+ *   at t + 1, sums in LDS and adds into the ring slot.  The library keeps a copy of the synapses
+ *   sorted per row by (target range, delay) with a u16 offset table, and the last D steps'
+ *   spike segments.  Needs plasticity NSX_PLAST_NONE, degrees below 65536 and (tiles of the
+ *   widest network) * (D - 1) <= 256; costs 8 bytes per synapse plus 2 bytes per (row, key) and
+ *   D * N * 4 bytes.  A batched population reads only its own networks' rows per tile. */
 #define NSX_DELIVERY_ATOMIC 0u
 #define NSX_DELIVERY_BINNED 1u
-/* tile_entries: 4096, 8192 or 16384 LDS accumulators per tile (0: automatic). */
+#define NSX_DELIVERY_PULL 2u
+/* tile_entries: 4096, 8192 or 16384 LDS accumulators per tile (0: automatic; BINNED only). */
 nsx_status nsx_set_delivery(nsx_ctx* ctx, uint32_t mode, uint64_t cap_events,
                             uint32_t tile_entries);
 nsx_status nsx_get_delivery(nsx_ctx* ctx, uint32_t* mode);
@@ -381,11 +391,12 @@ typedef struct nsx_tune_report {
   double best_us_per_step;
   double us_per_step[48];           /* per trial, in trial order                              */
   uint32_t trial_split[48], trial_graph[48];
-  uint32_t delivery, tile_entries;  /* chosen (tile_entries 0 with atomic delivery)           */
-  uint32_t trial_tile[48];          /* per trial: 0 atomic delivery, else binned tile entries */
+  uint32_t delivery, tile_entries;  /* chosen (tile entries: 0 atomic, 16384 pull)             */
+  uint32_t trial_tile[48];          /* per trial: binned tile entries, else 0                  */
   uint32_t row_split, syn_block;    /* chosen row-split threshold (0: whole rows), workgroup  */
   uint32_t update_width, _pad2;     /* chosen neurons per update thread (0: not a point model) */
   uint32_t trial_row_split[48], trial_syn_block[48], trial_width[48];
+  uint32_t trial_delivery[48];      /* per trial: NSX_DELIVERY_ATOMIC, _BINNED or _PULL         */
 } nsx_tune_report;
 
 nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* report);

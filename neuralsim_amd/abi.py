@@ -36,6 +36,7 @@ PLAST_TRIPLET = 2
 
 DELIVERY_ATOMIC = 0  # nsx_set_delivery: one 64-bit ring atomic per event
 DELIVERY_BINNED = 1  # events bucketed by (delay, target tile), summed in LDS at arrival
+DELIVERY_PULL = 2    # no plasticity: each target tile gathers its input from recent spikes
 # nsx_hh_spike_mode
 HH_SPIKE_CROSSING = 0
 HH_SPIKE_REFERENCE_RESET = 1
@@ -150,6 +151,7 @@ class NsxTuneReport(C.Structure):
         ("trial_row_split", C.c_uint32 * 48),
         ("trial_syn_block", C.c_uint32 * 48),
         ("trial_width", C.c_uint32 * 48),
+        ("trial_delivery", C.c_uint32 * 48),
     ]
 
 

@@ -652,6 +652,14 @@ nsx_status nsx_upload_weights(nsx_ctx* ctx, const float* weight, uint64_t m) {
   if (m != ctx->m) return set_error(ctx, NSX_E_INVALID, "weight count differs from the network's");
   nsx_status s = flush_pending(ctx, false);  // depressions pending on the old weights
   if (s) return s;
+  if (ctx->pull_built) {
+    // pull delivery: input owed by past spikes joins the ring with the weights it left with;
+    // the re-sorted copy is rebuilt with the new weights before the next step
+    if ((s = bin_materialize(ctx))) return s;
+    pull_release(ctx);
+    if (ctx->delivery == NSX_DELIVERY_PULL) ctx->bin_dirty = true;
+    destroy_graph(ctx);
+  }
   if (m)
     NSX_HIP(ctx, hipMemcpyAsync(ctx->w, weight, m * 4, hipMemcpyHostToDevice, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -251,6 +251,10 @@ static inline uint32_t blocks_for(uint64_t m, uint32_t b) {
 }
 
 static nsx_status free_network(nsx_ctx* ctx) {
+  // pull delivery: the input the old network's spikes still owe joins the ring while its rows
+  // are here; the tables belong to that network
+  if (ctx->p.pull_on) (void)pull_materialize(ctx);
+  pull_release(ctx);
   dev_free(ctx, ctx->row_ptr);
   dev_free(ctx, ctx->col);
   dev_free(ctx, ctx->w);
@@ -314,8 +318,9 @@ static void publish_network(nsx_ctx* ctx) {
   ctx->d.csc_src = ctx->csc_src;
   ctx->has_network = true;
   (void)flush_pending(ctx, false);  // depressions pending on the old weights are dropped
-  // binned delivery: events in flight join the ring; buffers are re-sized before the next step
-  if (ctx->delivery == NSX_DELIVERY_BINNED) {
+  // binned delivery: events in flight join the ring; buffers (or the pull tables) are re-sized
+  // before the next step
+  if (ctx->delivery != NSX_DELIVERY_ATOMIC) {
     (void)bin_materialize(ctx);
     ctx->bin_dirty = true;
   }

@@ -119,6 +119,30 @@ struct StepParams {
   // propagation with atomic delivery: rows longer than row_split slots are cut into pieces of at
   // most row_split slots, dealt round-robin to the block's prop_split wavefronts (0: whole rows)
   uint32_t row_split;
+  // pull delivery (networks without plasticity): the input of step t + 1 is gathered per target
+  // tile from the spikes of steps t + 2 - D .. t instead of pushed by a synapse pass
+  uint32_t pull_on, pull_ntiles;
+};
+
+// Pull delivery.  Each network's targets are cut into `parts` ranges of at most L neurons (L x
+// channels <= kPullTE LDS accumulators); every row keeps, besides the CSR, a copy of its synapses
+// sorted by (part of the target, delay) -- the key part * (D - 1) + d - 1 -- with the start of
+// each key's slice (entries [0, keys)) and the degree (entry keys) in a u16 offset table of
+// `keys + 1` entries per row.  A tile is one part of one network.
+constexpr uint32_t kPullTE = 16384;    // LDS int64 accumulators of a tile (channels x neurons)
+constexpr uint32_t kPullMaxKeys = 256;  // (part, delay) keys per row at most
+struct PullNet {
+  uint64_t doff_base;  // offset-table entry of the network's first row
+  uint32_t offset, n;  // first global neuron, neurons
+  uint32_t L, parts, stride;  // part length, parts, offset-table entries per row (keys + 1)
+  uint32_t _pad;
+};
+struct PullTile {
+  uint64_t doff_base;  // the network's doff_base
+  uint32_t lo, hi;     // target neurons [lo, hi) (global)
+  uint32_t slo, shi;   // source neurons: the network, [slo, shi)
+  uint32_t key0, stride;  // part * (D - 1); offset-table entries per row
+  uint32_t L, _pad;
 };
 
 // Epoch stepping (networks without plasticity whose shortest delay is d_min >= 2 steps): no spike
@@ -176,6 +200,14 @@ struct DevPtrs {
   uint32_t* bin_tab;   // [D][nbins + 1] bucket starts in that step's buffer (all 0: none)
   uint32_t* bin_hist;  // [groups][nbins] events per group and bucket, then group offsets
   uint32_t* bin_tot;   // [nbins] events per bucket
+  // pull delivery
+  const uint2* pdsyn;     // synapses re-sorted per row by (target part, delay): {col, w bits}
+  const uint16_t* pdoff;  // per row: start of each (part, delay) key's slice, then the degree
+  const PullTile* ptile;  // the pull kernel's tiles
+  uint32_t* phist_spk;    // [D][nblk * kBlock] spike segments of step e at slot e mod D
+  uint32_t* phist_row;    // [D][nblk * kBlock] the row start of each of those spikes
+  uint32_t* phist_cnt;    // [D][nblk] their counts
+  int32_t* pull_t0;       // first step whose spikes' input the ring does not hold
 };
 
 // Identity of a neuron for its RNG streams and E/I type: the network's seed and local index in
@@ -282,6 +314,7 @@ struct nsx_ctx {
   uint32_t epoch_alloc = 0;
   nsx::EpochSeg ep{nullptr, nullptr};
   bool bin_dirty = false;  // the network changed: buffers are re-sized before the next step
+  bool pull_built = false;  // the pull copy and tables match the loaded network and weights
   // nsx_snapshot_save / restore: one device copy of the mutable buffers, with the step index
   void* snap = nullptr;
   size_t snap_bytes = 0;
@@ -322,6 +355,15 @@ nsx_status bin_clear(nsx_ctx* ctx);
 nsx_status ring_mark(nsx_ctx* ctx, bool never);
 // (Re)allocate the bucket buffers for the current network, or release them.
 nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t tile_entries);
+// Pull delivery: build (or reuse) the re-sorted copy, tables and history for the loaded network;
+// free them; add every pending input into the ring and mark the history consumed; mark it
+// consumed only; launch the step's gather; refresh the copy's weights.
+nsx_status pull_configure(nsx_ctx* ctx);
+void pull_release(nsx_ctx* ctx);
+nsx_status pull_materialize(nsx_ctx* ctx);
+nsx_status pull_clear(nsx_ctx* ctx);
+hipError_t launch_pull(nsx_ctx* ctx, int32_t off);
+bool pull_supported(const nsx_ctx* ctx);
 // Shortest delay (steps) of the loaded network; the ring depth when it has no synapses.
 nsx_status network_min_delay(nsx_ctx* ctx, uint32_t* d_min);
 // Steps per epoch of the context as configured now (1: per-step launches).

@@ -1657,7 +1657,12 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
     hipLaunchKernelGGL(upd, dim3(gu), dim3(kBlock), 0, ctx->stream, p, ctx->d, off);
   }
   if (evs) (void)hipEventRecord(evs[1], ctx->stream);
-  if (p.bin_on) {
+  if (p.pull_on) {
+    if (evs) (void)hipEventRecord(evs[2], ctx->stream);
+    const hipError_t e = launch_pull(ctx, off);
+    if (e != hipSuccess) return e;
+    if (evs) (void)hipEventRecord(evs[3], ctx->stream);
+  } else if (p.bin_on) {
     const dim3 gb(p.bin_groups), bb(kBinThreads);
     if (pl == NSX_PLAST_PAIR)
       hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_PAIR>, gb, bb, 0, ctx->stream, p, ctx->d);
@@ -1814,6 +1819,7 @@ nsx_status flush_pending(nsx_ctx* ctx, bool apply) {
 // ---- binned delivery: host side ------------------------------------------------------------
 
 nsx_status bin_clear(nsx_ctx* ctx) {
+  if (ctx->p.pull_on) return pull_clear(ctx);
   if (!ctx->p.bin_on) return NSX_OK;
   NSX_HIP(ctx, hipMemsetAsync(ctx->d.bin_tab, 0,
                               (size_t)ctx->D * (ctx->p.bin_nbins + 1) * 4, ctx->stream));
@@ -1824,6 +1830,7 @@ nsx_status bin_clear(nsx_ctx* ctx) {
 // The arrival of step t (the next step) has run; the events of steps up to t - 1 that arrive
 // at t + 1 .. t + D - 2 are still in the buffers.
 nsx_status bin_materialize(nsx_ctx* ctx) {
+  if (ctx->p.pull_on) return pull_materialize(ctx);
   if (!ctx->p.bin_on) return NSX_OK;
   const int64_t t = (int64_t)ctx->t;
   for (int64_t ta = t + 1; t > 0 && ta <= t + ctx->D - 2; ++ta) {
@@ -1858,11 +1865,12 @@ static void bin_release(nsx_ctx* ctx) {
   ctx->d.bin_tab = ctx->d.bin_hist = ctx->d.bin_tot = nullptr;
   ctx->p.bin_on = 0;
   ctx->bin_te = 0;
+  ctx->p.pull_on = 0;  // the pull tables stay built for the network (pull_release frees them)
 }
 
 nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t te) {
-  if (mode != NSX_DELIVERY_ATOMIC && mode != NSX_DELIVERY_BINNED)
-    return set_error(ctx, NSX_E_INVALID, "delivery mode must be ATOMIC or BINNED");
+  if (mode != NSX_DELIVERY_ATOMIC && mode != NSX_DELIVERY_BINNED && mode != NSX_DELIVERY_PULL)
+    return set_error(ctx, NSX_E_INVALID, "delivery mode must be ATOMIC, BINNED or PULL");
   if (te != 0 && te != 4096 && te != 8192 && te != 16384)
     return set_error(ctx, NSX_E_INVALID, "tile_entries must be 0, 4096, 8192 or 16384");
   nsx_status s = bin_materialize(ctx);  // pending events move into the ring
@@ -1878,6 +1886,7 @@ nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t te)
     ctx->bin_dirty = true;
     return NSX_OK;
   }
+  if (mode == NSX_DELIVERY_PULL) return pull_configure(ctx);
   const uint32_t D = (uint32_t)ctx->D;
   const uint64_t m = ctx->m;
   if (m >= (1ull << 32)) return set_error(ctx, NSX_E_INVALID, "binned delivery needs M < 2^32");

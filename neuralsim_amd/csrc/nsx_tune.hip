@@ -131,14 +131,14 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   nsx_tune_report r{};
   double best = 1e300;
   struct Shape {
-    uint32_t split, graph, te, rs, nt, width;  // te 0: atomic delivery, else binned tile size
+    uint32_t split, graph, dl, te, rs, nt, width;  // dl: delivery; te: binned tile entries
   };
-  Shape cur{split0, graph0 ? graph0 : 50, deliv0 == NSX_DELIVERY_BINNED ? ctx->bin_te : 0u, rs0,
-            nt0, width0};
+  Shape cur{split0, graph0 ? graph0 : 50, deliv0,
+            deliv0 == NSX_DELIVERY_BINNED ? ctx->bin_te : 0u, rs0, nt0, width0};
   Shape bs = cur;
   auto trial = [&](const Shape& sh) -> nsx_status {
-    nsx_status st = bin_configure(ctx, sh.te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0,
-                                  sh.te);
+    nsx_status st =
+        bin_configure(ctx, sh.dl, cap0, sh.dl == NSX_DELIVERY_BINNED ? sh.te : 0u);
     if (!st) st = nsx_set_prop_split(ctx, sh.split);
     if (!st) st = nsx_set_row_split(ctx, sh.rs);
     if (!st) st = nsx_set_syn_block(ctx, sh.nt);
@@ -162,7 +162,8 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
       const uint32_t k = r.n_trials++;
       r.trial_split[k] = sh.split;
       r.trial_graph[k] = sh.graph;
-      r.trial_tile[k] = sh.te;
+      r.trial_tile[k] = sh.dl == NSX_DELIVERY_BINNED ? sh.te : 0u;
+      r.trial_delivery[k] = sh.dl;
       r.trial_row_split[k] = sh.rs;
       r.trial_syn_block[k] = sh.nt;
       r.trial_width[k] = sh.width;
@@ -188,7 +189,7 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     if (!s) s = trial(sh);
   }
   // 2. atomic delivery: the row-split threshold for high-degree rows, the workgroup size
-  if (!s && bs.te == 0) {
+  if (!s && bs.dl == NSX_DELIVERY_ATOMIC) {
     const Shape base = bs;
     for (uint32_t rs : {256u, 128u, 64u}) {
       Shape sh = base;
@@ -222,20 +223,26 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
       if (!s) s = trial(sh);
     }
   }
-  // 5. the other deliveries, and the split again under a delivery that wins
-  const uint32_t te_start = cur.te;
+  // 5. the other deliveries, and the split again under a push delivery that wins
+  const uint32_t dl_start = cur.dl, te_start = cur.te;
   if (!s) {
     const Shape base = bs;
-    for (uint32_t te : {0u, 8192u, 16384u}) {
-      if (s || te == te_start) continue;
-      if (te && !bin_supported(ctx, te)) continue;
+    static const uint32_t kDl[] = {NSX_DELIVERY_ATOMIC, NSX_DELIVERY_BINNED, NSX_DELIVERY_BINNED,
+                                   NSX_DELIVERY_PULL};
+    static const uint32_t kTe[] = {0u, 8192u, 16384u, 0u};
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t dl = kDl[k], te = kTe[k];
+      if (s || (dl == dl_start && te == te_start)) continue;
+      if (dl == NSX_DELIVERY_BINNED && !bin_supported(ctx, te)) continue;
+      if (dl == NSX_DELIVERY_PULL && !pull_supported(ctx)) continue;
       Shape sh = base;
+      sh.dl = dl;
       sh.te = te;
-      if (te) sh.rs = 0;  // binned passes walk whole rows
+      if (dl != NSX_DELIVERY_ATOMIC) sh.rs = 0;  // binned passes walk whole rows
       s = trial(sh);
     }
   }
-  if (!s && bs.te != te_start) {
+  if (!s && bs.dl != NSX_DELIVERY_PULL && (bs.dl != dl_start || bs.te != te_start)) {
     const Shape base = bs;
     for (uint32_t sp : kSplits) {
       Shape sh = base;
@@ -256,7 +263,7 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     (void)nsx_set_graph_steps(ctx, graph0);
     return s;
   }
-  if ((s = bin_configure(ctx, bs.te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC, cap0, bs.te)))
+  if ((s = bin_configure(ctx, bs.dl, cap0, bs.dl == NSX_DELIVERY_BINNED ? bs.te : 0u)))
     return s;
   if ((s = nsx_set_prop_split(ctx, bs.split)) || (s = nsx_set_row_split(ctx, bs.rs)) ||
       (s = nsx_set_syn_block(ctx, bs.nt)) || (s = nsx_set_update_width(ctx, bs.width)) ||
@@ -264,8 +271,8 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     return s;
   r.prop_split = bs.split;
   r.graph_steps = bs.graph;
-  r.delivery = bs.te ? NSX_DELIVERY_BINNED : NSX_DELIVERY_ATOMIC;
-  r.tile_entries = bs.te;
+  r.delivery = bs.dl;
+  r.tile_entries = bs.dl == NSX_DELIVERY_BINNED ? bs.te : bs.dl == NSX_DELIVERY_PULL ? kPullTE : 0u;
   r.row_split = bs.rs;
   r.syn_block = bs.nt;
   r.update_width = point ? bs.width : 0u;

@@ -200,9 +200,9 @@ class SimulationEngine:
         return e.value
 
     def set_delivery(self, mode, cap_events=0, tile_entries=0):
-        """How events reach the ring (abi.DELIVERY_ATOMIC / DELIVERY_BINNED); results are
-        bit-identical.  cap_events bounds one step's bucket buffer (more events: atomics that
-        step); tile_entries is 4096, 8192 or 16384 (0: automatic)."""
+        """How events reach the ring (abi.DELIVERY_ATOMIC / DELIVERY_BINNED / DELIVERY_PULL);
+        results are bit-identical.  cap_events bounds one step's bucket buffer (more events:
+        atomics that step); tile_entries is 4096, 8192 or 16384 (0: automatic), binned only."""
         self._chk(self._lib.nsx_set_delivery(self._h, int(mode), int(cap_events),
                                              int(tile_entries)), "nsx_set_delivery")
 
@@ -228,7 +228,8 @@ class SimulationEngine:
     def tune_trials(self):
         """Every trial of the last autotune as a dict."""
         r = self.last_tune
-        return [{"split": r.trial_split[k], "graph": r.trial_graph[k], "tile": r.trial_tile[k],
+        return [{"split": r.trial_split[k], "graph": r.trial_graph[k],
+                 "delivery": r.trial_delivery[k], "tile": r.trial_tile[k],
                  "row_split": r.trial_row_split[k], "syn_block": r.trial_syn_block[k],
                  "width": r.trial_width[k], "us_per_step": r.us_per_step[k]}
                 for k in range(r.n_trials)]

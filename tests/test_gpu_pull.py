@@ -11,6 +11,7 @@ oracle's FP32 mirror:
 * across read_state (input owed by past spikes joins the ring), set_step, switching modes
   mid-run, snapshots, weight uploads and replacing the network;
 * with the autotuner, which tries it and keeps results unchanged;
+* on edge networks: no synapses, a two-slot ring, networks of two to five neurons;
 * networks it cannot take (plasticity, too many tiles x delays) are refused and stay atomic.
 """
 import numpy as np
@@ -240,3 +241,25 @@ def test_pull_refuses_what_it_cannot_take(nsx):
             eng.step(1)
         assert eng.delivery == ATOMIC
         assert eng.step(20).steps_done == 20
+
+
+def test_pull_edge_networks(nsx):
+    """No synapses; a ring of two slots (every delay one step); networks of a few neurons."""
+    print("case pull edge networks", flush=True)
+    empty = configs.lif_network(n=3000, seed=6, drive_scale=60.0, k_in=0, record_raster=True,
+                                raster_capacity=1 << 20)
+    _same(_run(nsx, empty, 300, mode=ATOMIC), _run(nsx, empty, 300, graph=50))
+    fields = ["v", "u", "spike_count"]
+    base = configs.population_base(record_raster=True, raster_capacity=1 << 22)
+    base.ring_slots = 2
+    nets = configs.population_nets(6, seed=31, n_range=(200, 5000), delays=(1, 1))
+    sizes = [2, 3, 5, 300, 1200, 4000]
+    for b, n in zip(nets, sizes):
+        b.n_neurons, b.n_exc = n, max(1, n * 4 // 5)
+        b.k_in = min(b.k_in, n)
+    a = _batch_run(nsx, base, nets, 300, ATOMIC, 0, fields)
+    b = _batch_run(nsx, base, nets, 300, PULL, 25, fields)
+    assert a[0].spikes == b[0].spikes > 100 and a[0].events == b[0].events
+    assert all(np.array_equal(x, y) for x, y in zip(a[1], b[1]))
+    for k in a[3]:
+        assert np.array_equal(_bits(a[3][k]), _bits(b[3][k])), k

@@ -88,6 +88,28 @@ __device__ inline void stage_scalars(MlpScalars& dst, const MlpImage* img, const
     d[k] = k < kScalarHead ? a[k] : b[k - kScalarHead];
 }
 
+// The inference fragments of the image (w1f .. w3lf, contiguous in MlpImage), staged into LDS by
+// kernels that evaluate the MLP many times per block: a wavefront then reads each operand from
+// LDS at its use instead of waiting on a global (L1 / L2) load before each MFMA.
+struct __attribute__((aligned(16))) MlpInferW {
+  bf16x8 w1f[2][64];
+  bf16x8 w2f[8][64];
+  bf16x8 w3f[2][64];
+  bf16x8 w3lf[2][64];
+};
+static_assert(offsetof(MlpImage, w2f) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w2f) &&
+                  offsetof(MlpImage, w3f) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w3f) &&
+                  offsetof(MlpImage, w3lf) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w3lf),
+              "the inference fragments are contiguous in the image");
+
+// Cooperative copy by the whole block; the caller synchronises before use.
+__device__ inline void stage_infer_weights(MlpInferW& dst, const MlpImage* img) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  u32x4* d = reinterpret_cast<u32x4*>(&dst);
+  const u32x4* a = reinterpret_cast<const u32x4*>(img->w1f);
+  for (uint32_t k = threadIdx.x; k < sizeof(MlpInferW) / 16; k += blockDim.x) d[k] = a[k];
+}
+
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -140,26 +162,39 @@ __device__ inline __bf16 bf16_half(uint32_t pair, uint32_t hi) {
 }
 
 // relu of a 32x32 accumulator tile (m = h0 + row, n = s0 + column) as bf16 into an [s][h] tile
-// (sh, 8-byte stores) and, when given, an [h][s] tile (hs).
+// (sh) and, when given, an [h][s] tile (hs).  Lane l < 32 holds rows 8g..8g+3 of column l and
+// lane l + 32 rows 8g+4..8g+7; one swap of lane halves (v_permlane32_swap) between the register
+// groups g = 2j and 2j + 1 gives lane l rows 16j..16j+7 and lane l + 32 rows 16j+8..16j+15, so
+// each lane writes two 16-byte row pieces.  32 column-wise 8-byte stores with a 144-byte row
+// pitch fall on 16 bank pairs (two lanes each); 16-byte stores of 16 lanes cover the 64 banks.
 __device__ inline void store_relu(const f32x16& c, uint32_t h0, uint32_t s0, __bf16* sh,
                                   __bf16* hs, uint32_t lane) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
   const uint32_t s = s0 + (lane & 31);
+  u32x2 r[4];
 #pragma unroll
   for (uint32_t g = 0; g < 4; ++g) {
-    const u32x2 r = {relu_bf16x2(c[4 * g], c[4 * g + 1]), relu_bf16x2(c[4 * g + 2], c[4 * g + 3])};
+    r[g] = u32x2{relu_bf16x2(c[4 * g], c[4 * g + 1]), relu_bf16x2(c[4 * g + 2], c[4 * g + 3])};
     if (hs) {
 #pragma unroll
       for (uint32_t q = 0; q < 4; ++q)
-        hs[(h0 + c32_row(4 * g + q, lane)) * kLdA + s] = bf16_half(r[q >> 1], q & 1u);
+        hs[(h0 + c32_row(4 * g + q, lane)) * kLdA + s] = bf16_half(r[g][q >> 1], q & 1u);
     }
-    *reinterpret_cast<u32x2*>(sh + s * kLdA + h0 + c32_row(4 * g, lane)) = r;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < 2; ++j) {
+    const auto x = __builtin_amdgcn_permlane32_swap(r[2 * j][0], r[2 * j + 1][0], false, false);
+    const auto y = __builtin_amdgcn_permlane32_swap(r[2 * j][1], r[2 * j + 1][1], false, false);
+    *reinterpret_cast<u32x4*>(sh + s * kLdA + h0 + 16 * j + 8 * (lane >> 5)) =
+        u32x4{x[0], y[0], x[1], y[1]};
   }
 }
 
 // Layer 1 of a 64-sample tile: relu(W1 x + b1) as bf16 into act ([s][h1], row kLdA) and, when
 // given, an [h1][s] tile (hs).  The lane's input row (x hi words, lo words, 1, 1, 0...) passes
 // through elements 0..15 of act's row `lane`, which the outputs then overwrite.
-__device__ inline void layer1(const MlpImage& W, const float xn[kMlpIn], __bf16* act, __bf16* hs,
+template <class WT>
+__device__ inline void layer1(const WT& W, const float xn[kMlpIn], __bf16* act, __bf16* hs,
                               uint32_t lane) {
   float hi[kMlpIn], lo[kMlpIn];
 #pragma unroll
@@ -190,7 +225,8 @@ __device__ inline void layer1(const MlpImage& W, const float xn[kMlpIn], __bf16*
 
 // Layer-2 pre-activations: acc[mt*2+nt] (h2 tile mt, sample tile nt) = b2 + W2 a1^T, a1 as
 // [s][h1]; the bias initialises the accumulators.
-__device__ inline void layer2(const MlpImage& W, const float* b2, const __bf16* a1_sh,
+template <class WT>
+__device__ inline void layer2(const WT& W, const float* b2, const __bf16* a1_sh,
                               f32x16 acc[4], uint32_t lane) {
 #pragma unroll
   for (uint32_t mt = 0; mt < 2; ++mt) {
@@ -222,7 +258,8 @@ __device__ inline f32x4* out_slot(InferLds& L, uint32_t s) {
 
 // Normalised outputs of the calling lane's sample (normalised inputs xn).  Every lane of the
 // wavefront must call it (the MFMAs use all 64 lanes).
-__device__ inline void mlp_infer(const MlpImage& W, const MlpScalars& S, InferLds& L,
+template <class WT>
+__device__ inline void mlp_infer(const WT& W, const MlpScalars& S, InferLds& L,
                                  const float xn[kMlpIn], float y[kMlpOut], uint32_t lane) {
   layer1(W, xn, L.act, nullptr, lane);
   wave_lds_sync();

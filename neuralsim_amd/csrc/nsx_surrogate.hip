@@ -314,7 +314,8 @@ __global__ __launch_bounds__(1024) void k_surr_reduce(AdamArgs a) {
 // frozen-rate step's change of V), and whether x lay inside the training envelope
 // (otherwise nx is the plain step).  Every lane of the wavefront must call it; `live` lanes only
 // use the result.
-__device__ inline bool surr_step(const MlpImage& W, const MlpScalars& S, InferLds& L,
+template <class WT>
+__device__ inline bool surr_step(const WT& W, const MlpScalars& S, InferLds& L,
                                  const HHConst& c, uint32_t substeps, const float x[kMlpIn],
                                  bool live, float nx[4], float res[kMlpOut], uint32_t lane) {
   const MlpNorm& nm = S.norm;
@@ -495,6 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_epoch_surr(StepParams p, DevPtrs s, 
                                                        EpochSeg ep, int32_t off, uint32_t len) {
   __shared__ InferLds L[kBlock / 64];
   __shared__ MlpScalars S;
+  __shared__ MlpInferW WL;
   __shared__ uint32_t s_fb[kBlock / 64];
   __shared__ uint32_t s_drive[kBlock];
   const int32_t t0 = *s.tbase + off;
@@ -515,6 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_epoch_surr(StepParams p, DevPtrs s, 
     y = hh_syn_load<DRIVE>(s, i);
   }
   stage_scalars(S, q.img, q.norm);
+  stage_infer_weights(WL, q.img);
   const size_t plane = (size_t)p.ch * p.n;
   long long qa = 0, qb = 0;
   if (live && m0 >= t0 - (int32_t)p.D + 1) {
@@ -560,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_epoch_surr(StepParams p, DevPtrs s, 
       for (int kk = 0; kk < kMlpIn; ++kk) x[kk] = nm.in_mu[kk];
     }
     float nx[4], res[kMlpOut];
-    const bool in = surr_step(*q.img, S, L[wid], p.hh, p.hh_substeps, x, live, nx, res, lane);
+    const bool in = surr_step(WL, S, L[wid], p.hh, p.hh_substeps, x, live, nx, res, lane);
     bool spiked = false;
     if (live) {
       spiked = (x[0] < p.hh_spike_v) && (nx[0] >= p.hh_spike_v);

@@ -1,4 +1,4 @@
-"""Spike-train comparison used to judge the HH surrogate against a reference run.
+"""Spike-train comparison used to score the HH surrogate against a reference run.
 
 A spike of neuron i at step s is *displaced* when the other run has no spike of neuron i within
 `tol_ms` of it.  The displaced fraction pools both directions -- reference spikes the run under

@@ -3,7 +3,7 @@ the published model fix (CPU only):
 
 * standard-mode Hodgkin-Huxley -- the membrane equation of the reference's doc comment
   (hodgkin_huxley.rs:6-9), used by the 200k-neuron HH workload, the surrogate's training data
-  and its FP64 judge: resting potential, gate steady states from the rate closed forms, the
+  and its FP64 reference: resting potential, gate steady states from the rate closed forms, the
   rheobase of repetitive firing, the firing rate and spike shape at I = 10 uA/cm^2 (Hodgkin &
   Huxley 1952; the squid-axon constants of hodgkin_huxley.rs:98-114);
 * the Ornstein-Uhlenbeck drive: stationary mean and variance and the lag-k autocorrelation

@@ -255,8 +255,8 @@ nsx_status nsx_get_epoch(nsx_ctx* ctx, uint32_t* steps);
  *   step's buffer (two counting passes give each workgroup its offsets); at arrival one
  *   workgroup per tile sums its buckets in LDS and adds them into the ring slot.  Needs
  *   D - 1 <= 256, (D - 1) * tiles <= 8192 (tiles = ceil(N * channels / tile_entries)) and
- *   M < 2^32; a step with more events than cap_events (0: min(M, 2^24, 2^31 / (8 D)), at least
- *   min(M, 2^16)) falls back to atomics.  Costs D * cap_events * 8 bytes.  Chosen before a
+ *   M < 2^32; a step with more events than cap_events (0: min(M, 2^32 / (8 D), max(2^24, M / 8)),
+ *   at least min(M, 2^16)) falls back to atomics.  Costs D * cap_events * 8 bytes.  Chosen before a
  *   network is loaded, BINNED is sized at the next nsx_step; if that network does not meet the
  *   limits, the call fails with NSX_E_INVALID and the engine continues with ATOMIC.
  * PULL (networks without plasticity): no synapse pass.  After the update of step t, one

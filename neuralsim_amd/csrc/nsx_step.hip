@@ -1913,7 +1913,11 @@ nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t te)
   NSX_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBinThreads, 0));
   const uint32_t groups = (uint32_t)std::min(std::max(cus * std::max(per_cu, 1), 1), 1024);
   if (cap == 0) {
-    cap = std::min<uint64_t>({m, 1ull << 24, (1ull << 31) / (8ull * D)});
+    // room for bursts (a step over the cap delivers with the ring atomics): an eighth of all
+    // synapses in one step, at least 2^24 events, the D step buffers within 4 GB (the
+    // 256-network population bursts to 27 M events in a step, 4.7 times its mean)
+    const uint64_t mem_cap = (1ull << 32) / (8ull * D);
+    cap = std::min<uint64_t>({m, mem_cap, std::max<uint64_t>(1ull << 24, m / 8)});
     cap = std::max<uint64_t>(cap, std::min<uint64_t>(m, 1ull << 16));
   }
   cap = std::max<uint64_t>(std::min<uint64_t>(cap, 0xFFFFFFFFull), 1);

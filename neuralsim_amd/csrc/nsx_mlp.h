@@ -2,22 +2,31 @@
 // accumulation, evaluated and differentiated by one wavefront per chunk of 64 samples.
 //
 // All three layers run on gfx950 bf16 MFMA (v_mfma_f32_32x32x16_bf16 / v_mfma_f32_16x16x32_bf16)
-// with activations rounded to bf16 and staged through LDS.  Layer 1 (K = 5) carries each
-// normalised input as two bf16 words, x = hi + lo (lo = bf16(x - hi), 16 significant bits), and
-// its bias as two words against constant-1 inputs: K = 5 + 5 + 2 <= 16, one MFMA per 32x32 tile,
-// so neither the inputs nor the bias are cut to 8 bits.  Layer 2's bias initialises the
-// accumulators.  The output layer's weights are carried as two bf16 words, W3 = hi + lo, and
-// multiplied in two MFMA passes: its rounding error lands directly on the outputs, and a single
-// bf16 word there is the largest error of the whole surrogate step.  ReLU runs on the packed
-// bf16 results as a signed 16-bit max against 0 (the sign bit of a bf16 is that of an int16, so
-// relu-then-round and round-then-max agree).  Operand layouts:
+// with activations rounded to bf16.  Layer 1 (K = 5) carries each normalised input as two bf16
+// words, x = hi + lo (lo = bf16(x - hi), 16 significant bits), and its bias as two words against
+// constant-1 inputs: K = 5 + 5 + 2 <= 16, one MFMA per 32x32 tile, so neither the inputs nor the
+// bias are cut to 8 bits.  Layer 2's bias initialises the accumulators.  The output layer's
+// weights are carried as two bf16 words, W3 = hi + lo: its rounding error lands directly on the
+// outputs, and a single bf16 word there is the largest error of the whole surrogate step.  ReLU
+// runs on the packed bf16 results as a signed 16-bit max against 0 (the sign bit of a bf16 is
+// that of an int16, so relu-then-round and round-then-max agree).  Operand layouts:
 //   32x32x16: lane l holds A[m = l%32][k = 8(l/32)+j], B[k = 8(l/32)+j][n = l%32] (j < 8);
 //             C register i holds (m = 8(i/4) + 4(l/32) + i%4, n = l%32)
 //   16x16x32: lane l holds A[m = l%16][k = 8(l/16)+j], B[k = 8(l/16)+j][n = l%16];
 //             C register i holds (m = 4(l/16) + i, n = l%16)
-// Operand tiles are stored K-contiguous ([m][k] for A, [n][k] for B), so a lane's eight bf16 are
-// one 16-byte LDS read.  The weight operands are pre-arranged per lane in global memory (MlpImage)
-// so each wavefront loads them with coalesced 16-byte reads and no LDS.
+//
+// Inference (mlp_infer) keeps every activation in registers.  Samples are the n index of
+// 32x32x16 tiles throughout (lane l's sample is n = l%32 of tile l/32), so a layer's C tile is
+// the next layer's B operand as it stands: C registers 8q..8q+7 of the tile for output rows
+// 32mt.. hold, in lane half l/32, the 16 rows 16(2mt+q) + 8(j/4) + 4(l/32) + j%4 (j < 8) -- one
+// k-step of 16 rows in a fixed permuted order.  The next layer's A fragments store their K
+// columns in that order (w2pf, w3pf), so the contraction is unchanged and no activation passes
+// through LDS.  The layer-1 operand is each lane's own input row, moved to tile order by one swap
+// of lane halves; the output layer stacks W3's hi words (rows 0-3) and lo words (rows 4-7) in one
+// 32-row tile, whose hi and lo halves one more swap of lane halves brings to the sample's lane.
+// Training (mlp_train_chunk) stages operand tiles K-contiguous in LDS ([m][k] for A, [n][k] for
+// B), so a lane's eight bf16 are one 16-byte LDS read.  The weight operands are pre-arranged per
+// lane in global memory (MlpImage) and read with coalesced 16-byte loads.
 #pragma once
 #include <stddef.h>
 
@@ -54,9 +63,12 @@ constexpr int kNormFloats = (int)(sizeof(MlpNorm) / 4);
 constexpr int kL1Lo = kMlpIn, kL1Bias = 2 * kMlpIn;
 
 // Weights as the kernels read them (rebuilt from the FP32 masters after every optimiser step).
+// The inference fragments (w1f, w2pf, w3pf) come first and are contiguous.
 struct __attribute__((aligned(16))) MlpImage {
   float b2[kMlpH], b3[kMlpOut];  // FP32
   bf16x8 w1f[2][64];   // A fragments of [W1 | W1 | b1 hi | b1 lo | 0] [h1][16]: m-tile of 32
+  bf16x8 w2pf[8][64];  // A fragments of W2 [h2][h1], tile mt*4+ks, K in accumulator order
+  bf16x8 w3pf[4][64];  // A fragments of [W3 hi; W3 lo; 0] [32][h2], k-step ks, accumulator order
   bf16x8 w2f[8][64];   // A fragments of W2 [h2][h1]: tile mt*4+ks (m-tile of 32, k-step of 16)
   bf16x8 w3f[2][64];   // A fragments of W3 [o][h2] (hi words), o padded to 16: k-step of 32
   bf16x8 w3lf[2][64];  // the same for the lo words
@@ -64,8 +76,14 @@ struct __attribute__((aligned(16))) MlpImage {
   bf16x8 w3tf[2][64];  // A fragments of W3^T [h2][o] (hi words), o padded to 16: m-tile of 32
   bf16x8 w3tlf[2][64]; // the same for the lo words
 };
-constexpr int kImageFrags = 26;  // fragment arrays of 64 lanes
+constexpr int kImageFrags = 38;  // fragment arrays of 64 lanes
 static_assert(offsetof(MlpImage, w1f) % 16 == 0, "fragment alignment");
+
+// Hidden row held at element j (< 8) of k-step ks's operand in lane half `half`: the row order of
+// C registers 8(ks%2)..8(ks%2)+7 of a 32-row tile of rows 32(ks/2)..
+__host__ __device__ constexpr uint32_t acc_row(uint32_t ks, uint32_t half, uint32_t j) {
+  return 16u * ks + 8u * (j >> 2) + 4u * half + (j & 3u);
+}
 
 // The image's FP32 head and the normalisation, as staged into LDS by every block that evaluates
 // the MLP: read with wave-uniform addresses, they would otherwise be vector loads with a wait each
@@ -73,33 +91,42 @@ static_assert(offsetof(MlpImage, w1f) % 16 == 0, "fragment alignment");
 struct __attribute__((aligned(16))) MlpScalars {
   float b2[kMlpH], b3[kMlpOut];
   MlpNorm norm;
+  float b2p[kMlpH];  // b2 per (lane half, m-tile) in C-register order: [half*2 + mt][i]
 };
 constexpr int kScalarHead = kMlpH + kMlpOut;  // floats taken from the image
 constexpr int kScalarFloats = kScalarHead + kNormFloats;
-static_assert(offsetof(MlpImage, w1f) == kScalarHead * 4 && sizeof(MlpScalars) == kScalarFloats * 4,
-              "the image starts with b2, b3");
+static_assert(offsetof(MlpImage, w1f) == kScalarHead * 4 &&
+                  offsetof(MlpScalars, b2p) == kScalarFloats * 4 && kScalarFloats % 4 == 0,
+              "the image starts with b2, b3; b2p follows the normalisation, 16-byte aligned");
 
 // Cooperative copy by the whole block; the caller synchronises before use.
 __device__ inline void stage_scalars(MlpScalars& dst, const MlpImage* img, const MlpNorm* norm) {
   float* d = reinterpret_cast<float*>(&dst);
   const float* a = reinterpret_cast<const float*>(img);
   const float* b = reinterpret_cast<const float*>(norm);
-  for (int k = threadIdx.x; k < kScalarFloats; k += blockDim.x)
-    d[k] = k < kScalarHead ? a[k] : b[k - kScalarHead];
+  for (int k = threadIdx.x; k < kScalarFloats + kMlpH; k += blockDim.x) {
+    if (k < kScalarHead) {
+      d[k] = a[k];
+    } else if (k < kScalarFloats) {
+      d[k] = b[k - kScalarHead];
+    } else {  // b2p[(half*2 + mt)*16 + i] = b2[32mt + 8(i/4) + 4 half + i%4]
+      const int q = k - kScalarFloats, i = q & 15, mt = (q >> 4) & 1, half = q >> 5;
+      d[k] = a[32 * mt + 8 * (i >> 2) + 4 * half + (i & 3)];
+    }
+  }
 }
 
-// The inference fragments of the image (w1f .. w3lf, contiguous in MlpImage), staged into LDS by
+// The inference fragments of the image (w1f .. w3pf, contiguous in MlpImage), staged into LDS by
 // kernels that evaluate the MLP many times per block: a wavefront then reads each operand from
 // LDS at its use instead of waiting on a global (L1 / L2) load before each MFMA.
 struct __attribute__((aligned(16))) MlpInferW {
   bf16x8 w1f[2][64];
-  bf16x8 w2f[8][64];
-  bf16x8 w3f[2][64];
-  bf16x8 w3lf[2][64];
+  bf16x8 w2pf[8][64];
+  bf16x8 w3pf[4][64];
 };
-static_assert(offsetof(MlpImage, w2f) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w2f) &&
-                  offsetof(MlpImage, w3f) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w3f) &&
-                  offsetof(MlpImage, w3lf) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w3lf),
+static_assert(offsetof(MlpImage, w2pf) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w2pf) &&
+                  offsetof(MlpImage, w3pf) - offsetof(MlpImage, w1f) == offsetof(MlpInferW, w3pf) &&
+                  offsetof(MlpImage, w2f) - offsetof(MlpImage, w1f) == sizeof(MlpInferW),
               "the inference fragments are contiguous in the image");
 
 // Cooperative copy by the whole block; the caller synchronises before use.
@@ -247,52 +274,88 @@ __device__ inline void layer2(const WT& W, const float* b2, const __bf16* a1_sh,
     }
 }
 
-// LDS of one inference wavefront: [s][h] layer-1, then layer-2 activations; the 16 padding
-// bytes of row s (elements 64..71) receive sample s's four FP32 outputs.
-struct __attribute__((aligned(16))) InferLds {
-  __bf16 act[64 * kLdA];
-};
-__device__ inline f32x4* out_slot(InferLds& L, uint32_t s) {
-  return reinterpret_cast<f32x4*>(L.act + s * kLdA + 64);
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// relu of a 32x32 accumulator tile as bf16, packed as the next layer's two k-step operands:
+// op[q] holds C registers 8q..8q+7 (rows acc_row(2mt + q, lane half, j) of the tile's m-tile mt).
+__device__ inline void relu_operands(const f32x16& c, bf16x8 op[2]) {
+#pragma unroll
+  for (uint32_t q = 0; q < 2; ++q) {
+    const u32x4 r = {relu_bf16x2(c[8 * q], c[8 * q + 1]), relu_bf16x2(c[8 * q + 2], c[8 * q + 3]),
+                     relu_bf16x2(c[8 * q + 4], c[8 * q + 5]), relu_bf16x2(c[8 * q + 6], c[8 * q + 7])};
+    op[q] = __builtin_bit_cast(bf16x8, r);
+  }
 }
 
-// Normalised outputs of the calling lane's sample (normalised inputs xn).  Every lane of the
-// wavefront must call it (the MFMAs use all 64 lanes).
+// Normalised outputs of the calling lane's sample (normalised inputs xn), with every activation
+// in registers (layouts in the header comment).  Every lane of the wavefront must call it (the
+// MFMAs use all 64 lanes); no LDS is touched except the weights W (LDS or global) and S.
 template <class WT>
-__device__ inline void mlp_infer(const WT& W, const MlpScalars& S, InferLds& L,
-                                 const float xn[kMlpIn], float y[kMlpOut], uint32_t lane) {
-  layer1(W, xn, L.act, nullptr, lane);
-  wave_lds_sync();
-  f32x16 acc[4];
-  layer2(W, S.b2, L.act, acc, lane);
-  wave_lds_sync();  // every lane has read the layer-1 activations
+__device__ inline void mlp_infer(const WT& W, const MlpScalars& S, const float xn[kMlpIn],
+                                 float y[kMlpOut], uint32_t lane) {
+  const uint32_t half = lane >> 5;
+  float hi[kMlpIn], lo[kMlpIn];
 #pragma unroll
-  for (uint32_t mt = 0; mt < 2; ++mt)
-#pragma unroll
-    for (uint32_t nt = 0; nt < 2; ++nt)
-      store_relu(acc[mt * 2 + nt], mt * 32, nt * 32, L.act, nullptr, lane);
-  wave_lds_sync();
-  const bf16x8 w30 = W.w3f[0][lane], w31 = W.w3f[1][lane];
-  const bf16x8 w30l = W.w3lf[0][lane], w31l = W.w3lf[1][lane];
-#pragma unroll
-  for (uint32_t nt = 0; nt < 4; ++nt) {
-    f32x4 o = f32x4{};
-    mma16(o, w30l, L.act + nt * 16 * kLdA, kLdA, lane);
-    mma16(o, w31l, L.act + nt * 16 * kLdA + 32, kLdA, lane);
-    mma16(o, w30, L.act + nt * 16 * kLdA, kLdA, lane);
-    mma16(o, w31, L.act + nt * 16 * kLdA + 32, kLdA, lane);
-    if (lane < 16) {  // rows o = 0..3 of the output tile
-      f32x4 r;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) r[q] = o[q] + S.b3[q];
-      *out_slot(L, nt * 16 + lane) = r;  // padding bytes: no MFMA operand reads them
-    }
+  for (int k = 0; k < kMlpIn; ++k) {
+    hi[k] = (float)(__bf16)xn[k];
+    lo[k] = xn[k] - hi[k];  // exact; rounded to bf16 when packed
   }
-  wave_lds_sync();
-  const f32x4 r = *out_slot(L, lane);
+  // the lane's input row: columns 0-7 (hi 0-4, lo 0-2) and 8-15 (lo 3-4, 1, 1, 0...)
+  const uint32_t one = 0x3f803f80u;  // two bf16 1.0
+  const u32x4 r0 = {pack_bf16x2(hi[0], hi[1]), pack_bf16x2(hi[2], hi[3]),
+                    pack_bf16x2(hi[4], lo[0]), pack_bf16x2(lo[1], lo[2])};
+  const u32x4 r1 = {pack_bf16x2(lo[3], lo[4]), one, 0u, 0u};
+  // one swap of lane halves: x[0] = operand of samples 0..31 (lanes 32..63 carry columns 8-15
+  // of samples 0..31), x[1] = operand of samples 32..63
+  u32x4 xb[2];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) y[q] = r[q];
-  wave_lds_sync();
+  for (uint32_t q = 0; q < 4; ++q) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(r0[q], r1[q], false, false);
+    xb[0][q] = sw[0];
+    xb[1][q] = sw[1];
+  }
+  f32x16 out[2];
+#pragma unroll
+  for (uint32_t nt = 0; nt < 2; ++nt) {  // sample tile: samples 32nt .. 32nt+31
+    // one tile at a time: the tiles' chains need not overlap (other wavefronts fill the MFMA
+    // pipe), and keeping them apart holds the live registers to one tile's operands
+    if (nt) __builtin_amdgcn_sched_barrier(0);
+    const bf16x8 xo = __builtin_bit_cast(bf16x8, xb[nt]);
+    bf16x8 a1[4];  // layer-1 activations, k-steps of layer 2
+#pragma unroll
+    for (uint32_t mt = 0; mt < 2; ++mt)
+      relu_operands(__builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w1f[mt][lane], xo, f32x16{}, 0, 0, 0),
+                    a1 + 2 * mt);
+    bf16x8 a2[4];  // layer-2 activations, k-steps of layer 3
+#pragma unroll
+    for (uint32_t mt = 0; mt < 2; ++mt) {
+      f32x16 acc = *reinterpret_cast<const f32x16*>(S.b2p + (half * 2 + mt) * 16);
+#pragma unroll
+      for (uint32_t ks = 0; ks < 4; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w2pf[mt * 4 + ks][lane], a1[ks], acc, 0, 0,
+                                                      0);
+      relu_operands(acc, a2 + 2 * mt);
+    }
+    f32x16 o = f32x16{};
+#pragma unroll
+    for (uint32_t ks = 0; ks < 4; ++ks)
+      o = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W.w3pf[ks][lane], a2[ks], o, 0, 0, 0);
+    out[nt] = o;
+  }
+  // C registers 0..3: rows 0..3 (hi words' products) in lanes 0..31, rows 4..7 (lo words') in
+  // lanes 32..63, for sample 32nt + l%32; one swap gives lane l both parts of sample l.  Vector
+  // elements are copied to scalars before __builtin_bit_cast: this compiler's bit_cast of a
+  // vector element expression reads element 0.
+#pragma unroll
+  for (uint32_t q = 0; q < 4; ++q) {
+    const float e0 = out[0][q], e1 = out[1][q];
+    const auto sw = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, e0),
+                                                     __builtin_bit_cast(uint32_t, e1), false,
+                                                     false);
+    const uint32_t h = sw[0], l = sw[1];
+    float v = __builtin_bit_cast(float, h) + __builtin_bit_cast(float, l);
+    y[q] = v + S.b3[q];
+  }
 }
 
 // LDS of one training wavefront (forward and backward of a 64-sample chunk); 62,976 bytes.

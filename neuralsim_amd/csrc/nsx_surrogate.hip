@@ -156,46 +156,57 @@ __device__ inline __bf16 bf16_lo(float x) {  // the second word of a two-word bf
 __global__ void k_surr_pack(const float* w, MlpImage* img) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < (uint32_t)kImageFrags * 64u) {
-    const uint32_t f = t >> 6, l = t & 63u;
+    const uint32_t f = t >> 6, l = t & 63u, half = l >> 5;
     bf16x8 r;
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j) {
       __bf16 x = (__bf16)0.0f;
-      if (f < 8) {  // W2 [h2][h1]: m = h2 (tile mt), k = h1 (step ks)
-        const uint32_t mt = f >> 2, ks = f & 3u;
-        x = (__bf16)w[kOffW2 + (mt * 32 + (l & 31)) * 64 + ks * 16 + (l >> 5) * 8 + j];
-      } else if (f < 12) {  // W3 [o][h2] hi (8, 9) and lo (10, 11) words: m = o (< 4), k = h2
-        const uint32_t ks = (f - 8) & 1u, o = l & 15u;
-        if (o < 4) {
-          const float v = w[kOffW3 + o * 64 + ks * 32 + (l >> 4) * 8 + j];
-          x = f < 10 ? (__bf16)v : bf16_lo(v);
-        }
-      } else if (f < 20) {  // W2^T [h1][h2]: m = h1 (tile mt), k = h2 (step ks)
-        const uint32_t mt = (f - 12) >> 2, ks = (f - 12) & 3u;
-        const uint32_t h1 = mt * 32 + (l & 31), h2 = ks * 16 + (l >> 5) * 8 + j;
-        x = (__bf16)w[kOffW2 + h2 * 64 + h1];
-      } else if (f < 24) {  // W3^T [h2][o] hi (20, 21) and lo (22, 23) words: m = h2, k = o (< 4)
-        const uint32_t mt = (f - 20) & 1u, o = (l >> 5) * 8 + j;
-        if (o < 4) {
-          const float v = w[kOffW3 + o * 64 + mt * 32 + (l & 31)];
-          x = f < 22 ? (__bf16)v : bf16_lo(v);
-        }
-      } else {  // layer 1 [h1][k]: m = h1 (tile f - 24); k = input hi / lo column, bias words
-        const uint32_t h = (f - 24) * 32 + (l & 31), k = (l >> 5) * 8 + j;
+      if (f < 2) {  // layer 1 [h1][k]: m = h1 (tile f); k = input hi / lo column, bias words
+        const uint32_t h = f * 32 + (l & 31), k = half * 8 + j;
         if (k < (uint32_t)kL1Lo) x = (__bf16)w[kOffW1 + h * kMlpIn + k];
         else if (k < (uint32_t)kL1Bias) x = (__bf16)w[kOffW1 + h * kMlpIn + k - kL1Lo];
         else if (k == (uint32_t)kL1Bias) x = (__bf16)w[kOffB1 + h];
         else if (k == (uint32_t)kL1Bias + 1) x = bf16_lo(w[kOffB1 + h]);
+      } else if (f < 10) {  // W2 [h2][h1] for inference: k-step ks in accumulator row order
+        const uint32_t mt = (f - 2) >> 2, ks = (f - 2) & 3u;
+        x = (__bf16)w[kOffW2 + (mt * 32 + (l & 31)) * 64 + acc_row(ks, half, j)];
+      } else if (f < 14) {  // [W3 hi (rows 0-3); W3 lo (rows 4-7); 0] [32][h2], k-step f - 10
+        const uint32_t m = l & 31u;
+        if (m < 2 * kMlpOut) {
+          const float v = w[kOffW3 + (m & 3u) * 64 + acc_row(f - 10, half, j)];
+          x = m < (uint32_t)kMlpOut ? (__bf16)v : bf16_lo(v);
+        }
+      } else if (f < 22) {  // W2 [h2][h1]: m = h2 (tile mt), k = h1 (step ks)
+        const uint32_t mt = (f - 14) >> 2, ks = (f - 14) & 3u;
+        x = (__bf16)w[kOffW2 + (mt * 32 + (l & 31)) * 64 + ks * 16 + half * 8 + j];
+      } else if (f < 26) {  // W3 [o][h2] hi (22, 23) and lo (24, 25) words: m = o (< 4), k = h2
+        const uint32_t ks = (f - 22) & 1u, o = l & 15u;
+        if (o < 4) {
+          const float v = w[kOffW3 + o * 64 + ks * 32 + (l >> 4) * 8 + j];
+          x = f < 24 ? (__bf16)v : bf16_lo(v);
+        }
+      } else if (f < 34) {  // W2^T [h1][h2]: m = h1 (tile mt), k = h2 (step ks)
+        const uint32_t mt = (f - 26) >> 2, ks = (f - 26) & 3u;
+        const uint32_t h1 = mt * 32 + (l & 31), h2 = ks * 16 + half * 8 + j;
+        x = (__bf16)w[kOffW2 + h2 * 64 + h1];
+      } else {  // W3^T [h2][o] hi (34, 35) and lo (36, 37) words: m = h2, k = o (< 4)
+        const uint32_t mt = (f - 34) & 1u, o = half * 8 + j;
+        if (o < 4) {
+          const float v = w[kOffW3 + o * 64 + mt * 32 + (l & 31)];
+          x = f < 36 ? (__bf16)v : bf16_lo(v);
+        }
       }
       r[j] = x;
     }
-    if (f < 8) img->w2f[f][l] = r;
-    else if (f < 10) img->w3f[f - 8][l] = r;
-    else if (f < 12) img->w3lf[f - 10][l] = r;
-    else if (f < 20) img->w2tf[f - 12][l] = r;
-    else if (f < 22) img->w3tf[f - 20][l] = r;
-    else if (f < 24) img->w3tlf[f - 22][l] = r;
-    else img->w1f[f - 24][l] = r;
+    if (f < 2) img->w1f[f][l] = r;
+    else if (f < 10) img->w2pf[f - 2][l] = r;
+    else if (f < 14) img->w3pf[f - 10][l] = r;
+    else if (f < 22) img->w2f[f - 14][l] = r;
+    else if (f < 24) img->w3f[f - 22][l] = r;
+    else if (f < 26) img->w3lf[f - 24][l] = r;
+    else if (f < 34) img->w2tf[f - 26][l] = r;
+    else if (f < 36) img->w3tf[f - 34][l] = r;
+    else img->w3tlf[f - 36][l] = r;
     return;
   }
   const uint32_t u = t - (uint32_t)kImageFrags * 64u;
@@ -315,9 +326,9 @@ __global__ __launch_bounds__(1024) void k_surr_reduce(AdamArgs a) {
 // (otherwise nx is the plain step).  Every lane of the wavefront must call it; `live` lanes only
 // use the result.
 template <class WT>
-__device__ inline bool surr_step(const WT& W, const MlpScalars& S, InferLds& L,
-                                 const HHConst& c, uint32_t substeps, const float x[kMlpIn],
-                                 bool live, float nx[4], float res[kMlpOut], uint32_t lane) {
+__device__ inline bool surr_step(const WT& W, const MlpScalars& S, const HHConst& c,
+                                 uint32_t substeps, const float x[kMlpIn], bool live,
+                                 float nx[4], float res[kMlpOut], uint32_t lane) {
   const MlpNorm& nm = S.norm;
   bool in = live;
   float xn[kMlpIn];
@@ -328,7 +339,7 @@ __device__ inline bool surr_step(const WT& W, const MlpScalars& S, InferLds& L,
     xn[k] = d * nm.in_isd[k];
   }
   float y[kMlpOut];
-  mlp_infer(W, S, L, xn, y, lane);  // the MFMAs span the wavefront
+  mlp_infer(W, S, xn, y, lane);  // the MFMAs span the wavefront
 #pragma unroll
   for (int o = 0; o < kMlpOut; ++o) {
     float d = y[o] * nm.out_sd[o];
@@ -360,10 +371,9 @@ __global__ __launch_bounds__(256) void k_surr_forward(const MlpImage* img, const
                                                       HHConst c, uint32_t substeps,
                                                       const float* x, uint64_t count,
                                                       uint32_t what, float* y, uint8_t* inside) {
-  __shared__ InferLds L[4];
   __shared__ MlpScalars S;
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
   stage_scalars(S, img, norm);
   __syncthreads();
   const MlpNorm& nm = S.norm;
@@ -372,7 +382,7 @@ __global__ __launch_bounds__(256) void k_surr_forward(const MlpImage* img, const
 #pragma unroll
   for (int k = 0; k < kMlpIn; ++k) xs[k] = live ? x[i * kMlpIn + k] : nm.in_mu[k];
   float nx[4], res[kMlpOut];
-  const bool in = surr_step(*img, S, L[wid], c, substeps, xs, live, nx, res, lane);
+  const bool in = surr_step(*img, S, c, substeps, xs, live, nx, res, lane);
   if (live) {
     if (what == 2) {
       nx[0] = xs[0];
@@ -397,8 +407,8 @@ struct SurrDev {
 template <int DRIVE, bool PAIR>
 __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s, SurrDev q,
                                                         int32_t off) {
-  __shared__ InferLds L[kBlock / 64];
   __shared__ MlpScalars S;
+  __shared__ MlpInferW WL;
   __shared__ uint32_t s_fb[kBlock / 64];
   __shared__ uint32_t s_drive[kBlock];
   const int32_t t = *s.tbase + off;
@@ -423,6 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
     y = hh_syn_load<DRIVE>(s, i);
   }
   stage_scalars(S, q.img, q.norm);
+  stage_infer_weights(WL, q.img);  // one L2 read per block instead of one per wavefront
   drive_stage<DRIVE>(p, s, t, s_drive, id);
   __syncthreads();
   const MlpNorm& nm = S.norm;
@@ -440,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_update_surr(StepParams p, DevPtrs s,
     for (int k = 0; k < kMlpIn; ++k) x[k] = nm.in_mu[k];
   }
   float nx[4], res[kMlpOut];
-  const bool in = surr_step(*q.img, S, L[wid], p.hh, p.hh_substeps, x, live, nx, res, lane);
+  const bool in = surr_step(WL, S, p.hh, p.hh_substeps, x, live, nx, res, lane);
   bool spiked = false;
   if (live) {
     spiked = (x[0] < p.hh_spike_v) && (nx[0] >= p.hh_spike_v);
@@ -494,7 +505,6 @@ hipError_t launch_update_surrogate(nsx_ctx* ctx, int32_t off) {
 template <int DRIVE>
 __global__ __launch_bounds__(kBlock) void k_epoch_surr(StepParams p, DevPtrs s, SurrDev q,
                                                        EpochSeg ep, int32_t off, uint32_t len) {
-  __shared__ InferLds L[kBlock / 64];
   __shared__ MlpScalars S;
   __shared__ MlpInferW WL;
   __shared__ uint32_t s_fb[kBlock / 64];
@@ -563,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void k_epoch_surr(StepParams p, DevPtrs s, 
       for (int kk = 0; kk < kMlpIn; ++kk) x[kk] = nm.in_mu[kk];
     }
     float nx[4], res[kMlpOut];
-    const bool in = surr_step(WL, S, L[wid], p.hh, p.hh_substeps, x, live, nx, res, lane);
+    const bool in = surr_step(WL, S, p.hh, p.hh_substeps, x, live, nx, res, lane);
     bool spiked = false;
     if (live) {
       spiked = (x[0] < p.hh_spike_v) && (nx[0] >= p.hh_spike_v);

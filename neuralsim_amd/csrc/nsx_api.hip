@@ -267,6 +267,8 @@ static nsx_status check_device(nsx_ctx* ctx, int* dev) {
 }
 
 void destroy_graph(nsx_ctx* ctx) {
+  // no replay of the graph may still be in flight when its executable is destroyed
+  if (ctx->graph_exec) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   ctx->graph_exec = nullptr;

@@ -324,7 +324,8 @@ nsx_status nsx_weight_stats_noalloc(nsx_ctx* ctx, double* mean, double* stddev);
 /* Kernel timing: when enabled, every launch of the named kernel class in nsx_step is bracketed
  * by hipEvents on the stream it runs on (steps run as direct launches, not graphs).  Enabling
  * starts a fresh measurement; disabling keeps it readable.  kind: 0 neuron update, 1 synapse
- * pass (propagation and STDP), 2 binned delivery's counting passes, 3 its arrival pass; bytes
+ * pass (propagation and STDP), 2 binned delivery's bucket passes (the scan of the synapse
+ * pass's per-bucket counts and the placement of its records), 3 its arrival pass; bytes
  * report the STDP slots of the last call (1: at pre spikes, 2: at post spikes). */
 nsx_status nsx_set_profiling(nsx_ctx* ctx, int enable);
 nsx_status nsx_kernel_time(nsx_ctx* ctx, int kind, double* total_ms, uint64_t* launches,

@@ -941,7 +941,10 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   if (prof) {
     for (uint64_t k = 0; k < groups; ++k) {
       hipEvent_t* evs = &ctx->ev_pool[nev * k];
-      static const int kind_of[4] = {0, 2, 1, 3};  // event interval -> nsx_kernel_time kind
+      // event interval -> nsx_kernel_time kind: binned delivery runs update, synapse pass,
+      // bucket passes, arrival; the others update, (none), synapse pass (or pull), (none)
+      static const int kind_bin[4] = {0, 1, 2, 3}, kind_other[4] = {0, 2, 1, 3};
+      const int* kind_of = ctx->p.bin_on ? kind_bin : kind_other;
       for (size_t q = 0; q + 1 < nev; ++q) {
         const int kd = kind_of[q];
         if ((kd == 2 || kd == 3) && !ctx->p.bin_on) continue;

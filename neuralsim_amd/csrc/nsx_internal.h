@@ -113,6 +113,8 @@ struct StepParams {
   // binned delivery (bin_on = 1): tiles of 2^bin_tshift neurons, bucket (d - 1) * ntiles + tile
   uint32_t bin_on, bin_tshift, bin_ntiles, bin_nbins, bin_groups;
   uint32_t bin_cap;  // events one step's bucket buffer holds; more fall back to ring atomics
+  uint32_t bin_place_ch;  // records k_bin_place sorts per LDS chunk (a multiple of kBinThreads)
+  uint32_t bin_gcap;      // records of each synapse workgroup's scratch region per step
   // epoch stepping: several steps per update launch, one synapse pass for all of them; the
   // per-block counters are then added atomically (items of several steps share a block)
   uint32_t part_atomic;
@@ -200,6 +202,8 @@ struct DevPtrs {
   uint32_t* bin_tab;   // [D][nbins + 1] bucket starts in that step's buffer (all 0: none)
   uint32_t* bin_hist;  // [groups][nbins] events per group and bucket, then group offsets
   uint32_t* bin_tot;   // [nbins] events per bucket
+  uint2* bin_scr;      // [bin_cap] the step's records in synapse-pass order, one region per group
+  uint2* bin_reg;      // [groups] each group's region of bin_scr: {start, records}
   // pull delivery
   const uint2* pdsyn;     // synapses re-sorted per row by (target part, delay): {col, w bits}
   const uint16_t* pdoff;  // per row: start of each (part, delay) key's slice, then the degree
@@ -305,6 +309,7 @@ struct nsx_ctx {
   uint64_t bin_cap_req = 0;
   uint32_t bin_te_req = 0;  // tile entries requested (0: automatic)
   uint32_t bin_te = 0;      // tile entries in use
+  uint32_t bin_place_lds = 0;  // dynamic LDS bytes of k_bin_place
   uint32_t update_width = 0;  // neurons per thread of the point-model update (0: by size)
   uint32_t syn_block = 256;   // threads per workgroup of the (atomic-delivery) synapse pass
   // epoch stepping: steps per epoch at most (nsx_set_epoch; <= 1 off), the loaded network's

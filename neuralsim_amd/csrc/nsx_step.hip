@@ -546,18 +546,11 @@ __device__ inline float pair_flush(const StepParams& p, const DevPtrs& s, float 
   return w;
 }
 
-// Binned delivery of one event: its slot in the step's bucket buffer comes from the
-// workgroup's LDS cursor of bucket (d - 1, target tile); the record is the target's LDS entry
-// in its tile's accumulator (channel-major) and the delivered weight.
-// A slot that does not exist (live = false) takes no place: its cursor add is 0 and nothing is
-// stored, but its loads stay unconditional so that all slots' loads are in flight together.
-__device__ inline void deliver_bin(const StepParams& p, uint32_t* pos, uint2* ev, uint32_t d,
-                                   uint32_t j, uint32_t c, float w, bool live) {
-  const uint32_t bin = (d - 1u) * p.bin_ntiles + (j >> p.bin_tshift);
-  const uint32_t entry = (c << p.bin_tshift) | (j & ((1u << p.bin_tshift) - 1u));
-  const uint32_t k = atomicAdd(&pos[bin], live ? 1u : 0u);
-  if (live) ev[k] = make_uint2(entry, __float_as_uint(w));
-}
+// Binned delivery: a record is the event's bucket (d - 1, target tile) and the target's LDS
+// entry in its tile's accumulator (channel-major), packed as (bucket << kBinEntryBits) | entry,
+// with the delivered weight's bits.
+constexpr uint32_t kBinEntryBits = 14;  // tile entries <= 16384
+static_assert(kMaxBins << kBinEntryBits <= (1u << 27), "bucket and entry share one word");
 
 // The slots of one row chunk (chunk_setup done): propagation and the STDP at the pre spike.
 // Each lane takes kPropSlots slots per round, 64 apart, and issues all their loads before the
@@ -617,9 +610,26 @@ __device__ unsigned long long propagate_chunk(const StepParams& p, const DevPtrs
       uint32_t sl = tm + d[q];
       sl = sl >= p.D ? sl - p.D : sl;
       const uint32_t ch = p.ch > 1 ? rc.chan[rr[q]] : 0u;
+      bool atom = has[q];
       if (BIN && pos) {
-        deliver_bin(p, pos, bev, d[q], j[q], ch, w[q], has[q]);
-      } else if (has[q]) {
+        // appended to the workgroup's region of the step's scratch (pos[0] its cursor): one LDS
+        // add per wavefront and slot round, the round's records stored contiguously, and
+        // counted per bucket in pos[1 + bucket] (k_bin_place sorts them into their buckets).
+        // Lane 0 runs every round that any lane runs.  Past the region's end: ring atomics.
+        const unsigned long long bal = __ballot(has[q]);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(pos, (uint32_t)__popcll(bal));
+        base = __shfl(base, 0);
+        const uint32_t k = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (has[q] && k < p.bin_gcap) {
+          const uint32_t bin = (d[q] - 1u) * p.bin_ntiles + (j[q] >> p.bin_tshift);
+          const uint32_t entry = (ch << p.bin_tshift) | (j[q] & ((1u << p.bin_tshift) - 1u));
+          bev[k] = make_uint2((bin << kBinEntryBits) | entry, __float_as_uint(w[q]));
+          atomicAdd(&pos[1 + bin], 1u);
+          atom = false;
+        }
+      }
+      if (atom) {
         atomicAdd((unsigned long long*)(s.ring + sl * plane + (size_t)ch * p.n + j[q]),
                   (unsigned long long)quantize(w[q], p.scale_in));
       }
@@ -1203,42 +1213,17 @@ void k_synapses_epoch(StepParams p, DevPtrs s, EpochSeg ep, int32_t off, uint32_
 }
 
 // ---- binned delivery ------------------------------------------------------------------------
-// Per step t, after the update: (1) k_bin_hist counts each workgroup's propagation events per
-// bucket (delay, target tile) in LDS and writes its row of counts; (2) k_bin_scan turns the
-// [groups][buckets] counts into per-group offsets within each bucket and bucket totals;
-// (3) k_synapses_bin runs the synapse pass, each event stored at its bucket's start + its
-// group's offset + an LDS cursor; (4) k_bin_arrive, one workgroup per target tile, sums the
+// Per step t, after the update: (1) k_synapses_bin runs the synapse pass; each workgroup
+// appends its events' records to its own fixed-capacity region of the step's scratch (one LDS
+// cursor add per wavefront and round, stores contiguous) and counts them per bucket (delay,
+// target tile) in LDS, then writes its row of counts; events past its region's end take the
+// ring atomics; (2) k_bin_scan turns the [groups][buckets] counts into per-group offsets within
+// each bucket and bucket totals; (3) k_bin_place, one workgroup per region, sorts the region
+// into the buckets' positions of the step's event buffer (start_b + the group's offset) and
+// publishes the bucket starts; (4) k_bin_arrive, one workgroup per target tile, sums the
 // tile's buckets of every emission step whose events arrive at t + 1 into LDS int64
 // accumulators and adds them into ring slot (t + 1) mod D.  Integer sums do not depend on the
 // order of the events, so the ring equals the one atomic delivery builds.
-
-// Counting pass over the slots of one row chunk: four slots per lane per round; every load is
-// unconditional (a slot past the end repeats the first and counts 0) so that all eight are in
-// flight together.
-__device__ inline void hist_chunk(const StepParams& p, const DevPtrs& s, const RowChunk& rc,
-                                  uint32_t* h, uint32_t total, uint32_t lane) {
-  uint32_t r = 0;
-  for (uint32_t f = lane; f < total; f += 256) {
-    uint64_t k[4];
-    uint32_t add[4];
-    k[0] = chunk_slot(rc, f, r);
-    add[0] = 1u;
-#pragma unroll
-    for (uint32_t u = 1; u < 4; ++u) {
-      add[u] = f + 64u * u < total ? 1u : 0u;
-      k[u] = add[u] ? chunk_slot(rc, f + 64u * u, r) : k[0];
-    }
-    uint32_t j[4], d[4];
-#pragma unroll
-    for (uint32_t u = 0; u < 4; ++u) {
-      j[u] = s.col[k[u]];
-      d[u] = s.delay[k[u]];
-    }
-#pragma unroll
-    for (uint32_t u = 0; u < 4; ++u)
-      atomicAdd(&h[(d[u] - 1u) * p.bin_ntiles + (j[u] >> p.bin_tshift)], add[u]);
-  }
-}
 
 // Spiking rows of all the propagation items of a wavefront's batch (lane l: item (b, q, cnt),
 // `mine` when it is a propagation item whose segment part has rows), walked together in chunks
@@ -1292,26 +1277,6 @@ __device__ inline unsigned long long batch_rows(const StepParams& p, const DevPt
   return ev;
 }
 
-template <int PL>
-__global__ __launch_bounds__(kBinThreads) void k_bin_hist(StepParams p, DevPtrs s) {
-  __shared__ RowChunk sh[kBinThreads / 64];
-  __shared__ uint32_t h[kMaxBins];
-  const uint32_t nb = p.bin_nbins, lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  for (uint32_t k = threadIdx.x; k < nb; k += kBinThreads) h[k] = 0u;
-  __syncthreads();
-  RowChunk& rc = sh[wid];
-  for_batches<PL, kBinThreads>(
-      p, s,
-      [&](bool mine, uint32_t b, uint32_t q, uint32_t cnt) {
-        batch_rows(p, s, rc, lane, mine, b, q, cnt, false,
-                   [&](uint32_t total) { hist_chunk(p, s, rc, h, total, lane); });
-      },
-      [&](uint32_t, uint32_t, uint32_t) {});
-  __syncthreads();
-  uint32_t* row = s.bin_hist + (size_t)blockIdx.x * nb;
-  for (uint32_t k = threadIdx.x; k < nb; k += kBinThreads) row[k] = h[k];
-}
-
 // kScanCols buckets per workgroup of 256 threads: thread (column c, row group rg) holds its
 // column's counts of groups [rg * per, (rg + 1) * per) in registers (per <= kScanPer, groups <=
 // 1024), the row-group sums of a column are scanned in LDS, and the exclusive offsets are written
@@ -1351,71 +1316,24 @@ __global__ __launch_bounds__(kScanThreads) void k_bin_scan(StepParams p, DevPtrs
   if (rg == 0 && col < nb) s.bin_tot[col] = tot;
 }
 
-// Bucket starts of step t (exclusive scan of the totals, 8 buckets per thread) plus this
-// group's offsets, as LDS cursors.  Group 0 publishes the starts as the step's table; a step
-// whose events exceed the buffer publishes an empty table and returns false (atomic delivery).
-__device__ bool bin_positions(const StepParams& p, const DevPtrs& s, uint32_t* pos,
-                              uint32_t* wsum, int32_t t) {
-  const uint32_t nb = p.bin_nbins, tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
-  uint32_t v[8], sum = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < 8; ++k) {
-    const uint32_t b = tid * 8 + k;
-    v[k] = b < nb ? s.bin_tot[b] : 0u;
-    sum += v[k];
-  }
-  uint32_t incl = sum;
-#pragma unroll
-  for (uint32_t o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(incl, o);
-    if (lane >= o) incl += y;
-  }
-  if (lane == 63) wsum[wid] = incl;
-  __syncthreads();
-  uint32_t pre = 0, total = 0;
-#pragma unroll
-  for (uint32_t q = 0; q < kBinThreads / 64; ++q) {
-    const uint32_t x = wsum[q];
-    pre += q < wid ? x : 0u;
-    total += x;
-  }
-  uint32_t run = pre + incl - sum;
-  const bool use = total <= p.bin_cap;
-  uint32_t* tab = s.bin_tab + (size_t)((uint32_t)t % p.D) * (nb + 1);
-  const uint32_t* off = s.bin_hist + (size_t)blockIdx.x * nb;
-#pragma unroll
-  for (uint32_t k = 0; k < 8; ++k) {
-    const uint32_t b = tid * 8 + k;
-    if (b < nb) {
-      pos[b] = run + off[b];
-      if (blockIdx.x == 0) tab[b] = use ? run : 0u;
-    }
-    run += v[k];
-  }
-  if (blockIdx.x == 0 && tid == 0) tab[nb] = use ? total : 0u;
-  __syncthreads();
-  return use;
-}
-
 template <int PL>
 __global__ __launch_bounds__(kBinThreads)
 void k_synapses_bin(StepParams p, DevPtrs s, int32_t off) {
   __shared__ RowChunk sh[kBinThreads / 64];
-  __shared__ uint32_t pos[kMaxBins];
-  __shared__ uint32_t wsum[kBinThreads / 64];
+  __shared__ uint32_t cnt[1 + kMaxBins];  // [0] the region's append cursor, [1 + b] bucket b
   const int32_t t = *s.tbase + off;
-  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  const bool use = bin_positions(p, s, pos, wsum, t);
-  uint32_t* cur = use ? pos : nullptr;
-  uint2* bev = s.bin_ev + (size_t)((uint32_t)t % p.D) * p.bin_cap;
+  const uint32_t nb = p.bin_nbins, lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  for (uint32_t k = threadIdx.x; k <= nb; k += kBinThreads) cnt[k] = 0u;
+  __syncthreads();
+  uint2* scr = s.bin_scr + (size_t)blockIdx.x * p.bin_gcap;
   RowChunk& rc = sh[wid];
   for_batches<PL, kBinThreads>(
       p, s,
-      [&](bool mine, uint32_t b, uint32_t q, uint32_t cnt) {
+      [&](bool mine, uint32_t b, uint32_t q, uint32_t c) {
         unsigned long long ltp = 0;
         const unsigned long long ev = batch_rows(
-            p, s, rc, lane, mine, b, q, cnt, PL == NSX_PLAST_PAIR, [&](uint32_t total) {
-              ltp += propagate_chunk<PL, true>(p, s, rc, t, total, lane, cur, bev);
+            p, s, rc, lane, mine, b, q, c, PL == NSX_PLAST_PAIR, [&](uint32_t total) {
+              ltp += propagate_chunk<PL, true>(p, s, rc, t, total, lane, cnt, scr);
             });
         if (PL != NSX_PLAST_NONE) {
 #pragma unroll
@@ -1432,9 +1350,130 @@ void k_synapses_bin(StepParams p, DevPtrs s, int32_t off) {
           }
         }
       },
-      [&](uint32_t b, uint32_t q, uint32_t cnt) {
-        if (PL != NSX_PLAST_NONE) depress_item<PL>(p, s, rc, t, b, q, cnt, lane);
+      [&](uint32_t b, uint32_t q, uint32_t c) {
+        if (PL != NSX_PLAST_NONE) depress_item<PL>(p, s, rc, t, b, q, c, lane);
       });
+  __syncthreads();
+  uint32_t* row = s.bin_hist + (size_t)blockIdx.x * nb;
+  for (uint32_t k = threadIdx.x; k < nb; k += kBinThreads) row[k] = cnt[1 + k];
+  if (threadIdx.x == 0)
+    s.bin_reg[blockIdx.x] = make_uint2(blockIdx.x * p.bin_gcap, min(cnt[0], p.bin_gcap));
+}
+
+// Placement of step t's records: workgroup g moves its region of the scratch into the buckets'
+// positions of the step's event buffer, start_b + off_(g,b) + (its records of bucket b so far),
+// in chunks of bin_place_ch records: counted per bucket in LDS (each record's rank within its
+// bucket kept), the counts scanned into chunk starts, the records staged in bucket order and
+// copied out in that order, so each bucket's run is written contiguously.  Group 0 publishes
+// the step's bucket starts (every step).  Dynamic LDS: gpos[nb], cst[nb], the staged chunk.
+// PER records per thread and chunk: kPlacePer with one workgroup per CU (as many groups as CUs),
+// kPlacePer2 when the groups outnumber the CUs and two workgroups share each CU (64 VGPRs,
+// <= 80 KB of LDS each).
+constexpr uint32_t kPlacePer = 16, kPlacePer2 = 4;
+template <uint32_t PER>
+__global__ __launch_bounds__(kBinThreads)
+__attribute__((amdgpu_waves_per_eu(PER > kPlacePer2 ? 4 : 8)))
+void k_bin_place(StepParams p, DevPtrs s, int32_t off) {
+  extern __shared__ uint32_t place_lds[];
+  __shared__ uint32_t wsum[kBinThreads / 64];
+  const int32_t t = *s.tbase + off;
+  const uint32_t nb = p.bin_nbins, g = blockIdx.x, tid = threadIdx.x;
+  const uint32_t lane = tid & 63u, wid = tid >> 6;
+  const uint32_t slot = (uint32_t)t % p.D;
+  uint32_t* gpos = place_lds;
+  uint32_t* cst = place_lds + nb;
+  uint2* stage = reinterpret_cast<uint2*>(place_lds + 2 * nb);  // 8 nb bytes: 8-byte aligned
+  const uint32_t* offr = s.bin_hist + (size_t)g * nb;
+  // exclusive scan of 8 values per thread over the workgroup (wsum: the waves' totals); returns
+  // the thread's exclusive prefix and the total
+  auto scan8 = [&](const uint32_t v[8], uint32_t& total) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) sum += v[k];
+    uint32_t incl = sum;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    total = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kBinThreads / 64; ++q) {
+      run += q < wid ? wsum[q] : 0u;
+      total += wsum[q];
+    }
+    __syncthreads();  // wsum is read before the next scan writes it
+    return run;
+  };
+  {  // bucket starts; this group's positions; group 0 publishes the table, every step
+    uint32_t v[8], total = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) v[k] = tid * 8 + k < nb ? s.bin_tot[tid * 8 + k] : 0u;
+    uint32_t run = scan8(v, total);
+    uint32_t* tab = s.bin_tab + (size_t)slot * (nb + 1);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      const uint32_t b = tid * 8 + k;
+      if (b < nb) {
+        gpos[b] = run + offr[b];
+        if (g == 0) tab[b] = run;
+      }
+      run += v[k];
+    }
+    if (g == 0 && tid == 0) tab[nb] = total;
+  }
+  const uint2 reg = s.bin_reg[g];
+  const uint2* src = s.bin_scr + reg.x;
+  uint2* ev = s.bin_ev + (size_t)slot * p.bin_cap;
+  const uint32_t CH = p.bin_place_ch, mask = (1u << kBinEntryBits) - 1u;
+  for (uint32_t c0 = 0; c0 < reg.y; c0 += CH) {
+    const uint32_t n = min(CH, reg.y - c0);
+    for (uint32_t b = tid; b < nb; b += kBinThreads) cst[b] = 0u;
+    __syncthreads();
+    uint2 x[PER];
+    uint32_t rk[PER];
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+      const uint32_t i = u * kBinThreads + tid;
+      x[u] = i < n ? src[c0 + i] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+      const uint32_t i = u * kBinThreads + tid;
+      rk[u] = i < n ? atomicAdd(&cst[x[u].x >> kBinEntryBits], 1u) : 0u;
+    }
+    __syncthreads();
+    {  // the chunk's counts -> chunk starts, in place
+      uint32_t c[8], total = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) c[k] = tid * 8 + k < nb ? cst[tid * 8 + k] : 0u;
+      uint32_t run = scan8(c, total);
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) {
+        if (tid * 8 + k < nb) cst[tid * 8 + k] = run;
+        run += c[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u) {
+      const uint32_t i = u * kBinThreads + tid;
+      if (i < n) stage[cst[x[u].x >> kBinEntryBits] + rk[u]] = x[u];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kBinThreads) {
+      const uint2 y = stage[i];
+      const uint32_t b = y.x >> kBinEntryBits;
+      ev[gpos[b] + (i - cst[b])] = make_uint2(y.x & mask, y.y);
+    }
+    __syncthreads();
+    for (uint32_t b = tid; b < nb; b += kBinThreads)
+      gpos[b] += (b + 1u < nb ? cst[b + 1] : n) - cst[b];
+    __syncthreads();  // the next chunk clears cst
+  }
 }
 
 // Arrival at step ta = *tbase + off of the events emitted at steps e in [ta - D + 1, ta - 1]
@@ -1663,17 +1702,8 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
     if (e != hipSuccess) return e;
     if (evs) (void)hipEventRecord(evs[3], ctx->stream);
   } else if (p.bin_on) {
+    // events: update | synapse pass | count scan and placement | arrival (kinds 0, 1, 2, 3)
     const dim3 gb(p.bin_groups), bb(kBinThreads);
-    if (pl == NSX_PLAST_PAIR)
-      hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_PAIR>, gb, bb, 0, ctx->stream, p, ctx->d);
-    else if (pl == NSX_PLAST_TRIPLET)
-      hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_TRIPLET>, gb, bb, 0, ctx->stream, p, ctx->d);
-    else
-      hipLaunchKernelGGL(k_bin_hist<NSX_PLAST_NONE>, gb, bb, 0, ctx->stream, p, ctx->d);
-    hipLaunchKernelGGL(k_bin_scan, dim3((p.bin_nbins + kScanCols - 1) / kScanCols),
-                       dim3(kScanThreads), 0,
-                       ctx->stream, p, ctx->d);
-    if (evs) (void)hipEventRecord(evs[2], ctx->stream);
     if (pl == NSX_PLAST_PAIR)
       hipLaunchKernelGGL(k_synapses_bin<NSX_PLAST_PAIR>, gb, bb, 0, ctx->stream, p, ctx->d, off);
     else if (pl == NSX_PLAST_TRIPLET)
@@ -1681,6 +1711,15 @@ hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
                          off);
     else
       hipLaunchKernelGGL(k_synapses_bin<NSX_PLAST_NONE>, gb, bb, 0, ctx->stream, p, ctx->d, off);
+    if (evs) (void)hipEventRecord(evs[2], ctx->stream);
+    hipLaunchKernelGGL(k_bin_scan, dim3((p.bin_nbins + kScanCols - 1) / kScanCols),
+                       dim3(kScanThreads), 0, ctx->stream, p, ctx->d);
+    if (p.bin_place_ch > kPlacePer2 * kBinThreads)
+      hipLaunchKernelGGL(k_bin_place<kPlacePer>, gb, bb, ctx->bin_place_lds, ctx->stream, p,
+                         ctx->d, off);
+    else
+      hipLaunchKernelGGL(k_bin_place<kPlacePer2>, gb, bb, ctx->bin_place_lds, ctx->stream, p,
+                         ctx->d, off);
     if (evs) (void)hipEventRecord(evs[3], ctx->stream);
     const hipError_t e = launch_arrive(ctx, off + 1, INT32_MAX);
     if (e != hipSuccess) return e;
@@ -1861,7 +1900,9 @@ static void bin_release(nsx_ctx* ctx) {
   dev_free(ctx, ctx->d.bin_tab);
   dev_free(ctx, ctx->d.bin_hist);
   dev_free(ctx, ctx->d.bin_tot);
-  ctx->d.bin_ev = nullptr;
+  dev_free(ctx, ctx->d.bin_scr);
+  dev_free(ctx, ctx->d.bin_reg);
+  ctx->d.bin_ev = ctx->d.bin_scr = ctx->d.bin_reg = nullptr;
   ctx->d.bin_tab = ctx->d.bin_hist = ctx->d.bin_tot = nullptr;
   ctx->p.bin_on = 0;
   ctx->bin_te = 0;
@@ -1913,9 +1954,10 @@ nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t te)
   NSX_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBinThreads, 0));
   const uint32_t groups = (uint32_t)std::min(std::max(cus * std::max(per_cu, 1), 1), 1024);
   if (cap == 0) {
-    // room for bursts (a step over the cap delivers with the ring atomics): an eighth of all
-    // synapses in one step, at least 2^24 events, the D step buffers within 4 GB (the
-    // 256-network population bursts to 27 M events in a step, 4.7 times its mean)
+    // room for bursts (each synapse workgroup's region holds cap / groups records of a step;
+    // its events past that take the ring atomics): an eighth of all synapses in one step, at
+    // least 2^24 events, the D step buffers within 4 GB (the 256-network population bursts to
+    // 27 M events in a step, 4.7 times its mean)
     const uint64_t mem_cap = (1ull << 32) / (8ull * D);
     cap = std::min<uint64_t>({m, mem_cap, std::max<uint64_t>(1ull << 24, m / 8)});
     cap = std::max<uint64_t>(cap, std::min<uint64_t>(m, 1ull << 16));
@@ -1925,16 +1967,42 @@ nsx_status bin_configure(nsx_ctx* ctx, uint32_t mode, uint64_t cap, uint32_t te)
   if ((s = dev_alloc(ctx, (void**)&d.bin_ev, (size_t)D * cap * 8, false)) ||
       (s = dev_alloc(ctx, (void**)&d.bin_tab, (size_t)D * (nbins + 1) * 4, true)) ||
       (s = dev_alloc(ctx, (void**)&d.bin_hist, (size_t)groups * nbins * 4, true)) ||
-      (s = dev_alloc(ctx, (void**)&d.bin_tot, (size_t)nbins * 4, true))) {
+      (s = dev_alloc(ctx, (void**)&d.bin_tot, (size_t)nbins * 4, true)) ||
+      (s = dev_alloc(ctx, (void**)&d.bin_scr, (size_t)cap * 8, false)) ||
+      (s = dev_alloc(ctx, (void**)&d.bin_reg, (size_t)groups * 8, true))) {
     bin_release(ctx);
     return s;
   }
+  // placement chunks: all groups resident at once, so one workgroup per CU takes the rest of
+  // the CU's 160 KB of LDS beside gpos and cst (up to kPlacePer records per thread; a region's
+  // records of a step usually fit one chunk), and with more groups than CUs two share a CU
+  // (80 KB each, up to kPlacePer2 records per thread)
+  const bool two = groups > (uint32_t)cus;
+  const uint64_t room = (two ? (80u << 10) : (160u << 10)) - 8ull * nbins - 128u;
+  const uint32_t ch = (uint32_t)std::min<uint64_t>(
+      (two ? kPlacePer2 : kPlacePer) * kBinThreads,
+      std::max<uint64_t>(room / 8 / kBinThreads, 1) * kBinThreads);
+  const uint32_t place_lds = (uint32_t)(8ull * nbins + 8ull * ch);
+  {
+    const void* kp = ch > kPlacePer2 * kBinThreads ? (const void*)k_bin_place<kPlacePer>
+                                                   : (const void*)k_bin_place<kPlacePer2>;
+    const hipError_t e =
+        hipFuncSetAttribute(kp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)place_lds);
+    if (e != hipSuccess) {
+      bin_release(ctx);
+      return hip_error(ctx, e, "binned delivery placement LDS");
+    }
+  }
+  ctx->bin_place_lds = place_lds;
   StepParams& p = ctx->p;
   p.bin_tshift = tshift;
   p.bin_ntiles = (uint32_t)ntiles;
   p.bin_nbins = (uint32_t)nbins;
   p.bin_groups = groups;
   p.bin_cap = (uint32_t)cap;
+  // a workgroup's events of a step past its region (cap / groups records) take the ring atomics
+  p.bin_gcap = (uint32_t)(cap / groups);
+  p.bin_place_ch = ch;
   p.bin_on = 1;
   ctx->bin_te = te;
   return NSX_OK;

@@ -62,7 +62,16 @@ def synapse_bytes(events, spikes, ltp_slots, ltd_slots):
 def regime_tag(args):
     return {"workload": "izhikevich_pair_stdp_1M_100M", "neurons": args.neurons,
             "burnin_ms": args.burnin_ms, "step_ms": args.step_ms, "warmup": args.warmup,
-            "steps": args.steps}
+            "steps": args.steps, "delivery": args.delivery}
+
+
+def set_headline_delivery(ns, eng, delivery):
+    """Delivery of the headline network: binned (the default, measured faster on this network)
+    or ring atomics; "auto" leaves it to the autotuner, run by the caller."""
+    if delivery == "binned":
+        eng.set_delivery(ns.abi.DELIVERY_BINNED, 0, 0)
+    elif delivery == "atomic":
+        eng.set_delivery(ns.abi.DELIVERY_ATOMIC)
 
 
 def measured_traffic(kernel, tag):
@@ -123,9 +132,18 @@ def bench_headline(ns, args):
     log(f"bench: {spec.name} n={spec.n} m={spec.m} built on device in {build_s:.2f}s")
     if args.split:
         eng.set_prop_split(args.split)
+    set_headline_delivery(ns, eng, args.delivery)
     G = args.graph
     eng.set_graph_steps(G)
     eng.step(int(round(args.burnin_ms / dt)))
+    tuned = None
+    if args.delivery == "auto":
+        eng.autotune(200)
+        r = eng.last_tune
+        tuned = {"prop_split": int(r.prop_split), "delivery": ("atomic", "binned", "pull")[
+            int(eng.delivery)], "tile_entries": int(r.tile_entries), "row_split": int(r.row_split),
+            "syn_block": int(r.syn_block), "update_width": int(r.update_width)}
+        eng.set_graph_steps(G)
     eng.step(args.warmup * per)
     log(f"bench: burn-in {args.burnin_ms:g} ms + warmup {args.warmup} x {args.step_ms:g} ms done")
     K = args.steps
@@ -162,6 +180,7 @@ def bench_headline(ns, args):
             "burnin_ms": args.burnin_ms,
             "t_start_ms": args.burnin_ms + args.warmup * per * dt,
             "sim_ms": sim_s * 1e3, "accumulate": "int64 fixed point 2^-32",
+            "delivery": args.delivery if tuned is None else {"auto": tuned},
             "launch": ({"graph_steps": Gt, "graph_launches": S // Gt, "direct_steps": S % Gt}
                        if Gt else {"graph_steps": 0, "graph_launches": 0, "direct_steps": S}),
         },
@@ -181,28 +200,40 @@ def bench_headline(ns, args):
         pst = eng.step(S)
         eng.set_profiling(False)
         assert pst.events == st.events and pst.spikes == st.spikes, "replay differs"
-        kinds = {0: "k_update", 1: "k_synapses"}
+        binned = int(eng.delivery) == ns.abi.DELIVERY_BINNED
+        syn = "k_synapses_bin" if binned else "k_synapses"
+        kinds = {0: "k_update", 1: syn}
+        if binned:
+            kinds.update({2: "k_bin_scan+k_bin_place", 3: "k_bin_arrive"})
         times = {k: eng.kernel_time(k) for k in kinds}
         ltp_slots, ltd_slots = times[1][2], eng.kernel_time(2)[2]
         inv = 1.0 / S
+        ev = st.events * inv
         upd_b = neuron_bytes(spec.cfg, spec.n, st.spikes * inv)
-        prop_b, syn_b = synapse_bytes(st.events * inv, st.spikes * inv, ltp_slots * inv,
-                                      ltd_slots * inv)
+        # the 8 B per event of the synapse pass is its ring atomic (atomic delivery) or its
+        # record store (binned)
+        prop_b, syn_b = synapse_bytes(ev, st.spikes * inv, ltp_slots * inv, ltd_slots * inv)
         kt = {kinds[k]: {"avg_us": (ms / max(n, 1)) * 1e3, "launches": n}
               for k, (ms, n, _) in times.items()}
         kt["k_update"]["alg_bytes_per_launch"] = upd_b
-        kt["k_synapses"]["alg_bytes_per_launch"] = syn_b
-        kt["k_synapses"]["propagation_bytes_per_launch"] = prop_b
-        kt["k_synapses"]["events_per_launch"] = st.events * inv
-        kt["k_synapses"]["ltp_slots_per_launch"] = ltp_slots * inv
-        kt["k_synapses"]["ltd_slots_per_launch"] = ltd_slots * inv
+        kt[syn]["alg_bytes_per_launch"] = syn_b
+        kt[syn]["propagation_bytes_per_launch"] = prop_b
+        kt[syn]["events_per_launch"] = ev
+        kt[syn]["ltp_slots_per_launch"] = ltp_slots * inv
+        kt[syn]["ltd_slots_per_launch"] = ltd_slots * inv
+        if binned:
+            # placement reads and writes each 8 B record; the arrival reads it (a lower bound:
+            # the ring entries it adds into are not counted)
+            kt["k_bin_scan+k_bin_place"]["alg_bytes_per_launch"] = 16.0 * ev
+            kt["k_bin_arrive"]["alg_bytes_per_launch"] = 8.0 * ev
         for k in kinds.values():
             kt[k]["achieved_GBs"] = (kt[k]["alg_bytes_per_launch"]
                                      / (max(kt[k]["avg_us"], 1e-9) * 1e-6) / 1e9)
-        # one 64-bit ring atomic per event: rate against the device's measured random-atomic
-        # throughput (profiles/atomic_ceiling.md)
-        kt["k_synapses"]["atomics_G_per_s"] = st.events * inv / (kt["k_synapses"]["avg_us"] * 1e3)
-        kt["k_synapses"]["atomic_ceiling_G_per_s"] = 23.0
+        if not binned:
+            # one 64-bit ring atomic per event: rate against the device's measured random-atomic
+            # throughput (profiles/atomic_ceiling.md)
+            kt[syn]["atomics_G_per_s"] = ev / (kt[syn]["avg_us"] * 1e3)
+            kt[syn]["atomic_ceiling_G_per_s"] = 23.0
         dom = max(kinds.values(), key=lambda k: kt[k]["avg_us"])
         out["roofline"] = {
             "kernel": dom, "bound": "hbm", "achieved": kt[dom]["achieved_GBs"],
@@ -591,6 +622,8 @@ def main():
     ap.add_argument("--graph", type=int, default=50, help="steps per captured hipGraph (0: off)")
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel replay")
     ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
+    ap.add_argument("--delivery", choices=("binned", "atomic", "auto"), default="binned",
+                    help="spike delivery of the headline network (auto: the autotuner's choice)")
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU-port stepping per workload sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -622,6 +655,7 @@ def main():
         per = int(round(args.step_ms / spec.cfg.dt_ms))
         before = int(round(args.burnin_ms / spec.cfg.dt_ms)) + args.warmup * per
         with ns.SimulationEngine.from_spec(spec) as eng:
+            set_headline_delivery(ns, eng, args.delivery)
             eng.set_graph_steps(0)
             eng.step(before)
             st = eng.step(args.steps * per)

@@ -457,7 +457,7 @@ def bench_population(ns, steps=2000, warm=200, tune_steps=100, cpu=False, budget
            "tuned": {"prop_split": split, "graph_steps": graph,
                      "delivery": ("atomic", "binned", "pull")[deliv], "tile_entries": tile,
                      "row_split": row_split, "syn_block": syn_block, "update_width": width,
-                     "trials": trials},
+                     "persistent": int(r.persistent), "trials": trials},
            "timed_steps": steps - warm,
            "replay_kernel_us_per_step": kt,
            "ms_per_step": wall * 1e3 / (steps - warm),

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define NSX_ABI_VERSION 2u
+#define NSX_ABI_VERSION 3u
 
 /* GpuError {NoAdapter, DeviceRequest} (gpu_backend.rs:24-39) widened to every failure the
  * device path can report; maps onto NeuralSimError::Gpu (src/error.rs:44-46). */
@@ -247,6 +247,19 @@ nsx_status nsx_set_syn_block(nsx_ctx* ctx, uint32_t threads);
 nsx_status nsx_set_epoch(nsx_ctx* ctx, uint32_t max_steps);
 nsx_status nsx_get_epoch(nsx_ctx* ctx, uint32_t* steps);
 
+/* Persistent stepping (off by default; no reference counterpart: it replaces the per-step
+ * launches of simulate_ms's loop, engine.rs:792-797, and of the population's evaluation,
+ * population.rs:116-124).  A window of steps -- graph_steps of them, 1024 when graphs are off
+ * -- runs as one cooperative launch of a grid resident at once: per step the update of every
+ * neuron block, a grid barrier, the synapse pass with ring atomics, a grid barrier.  Applies
+ * with ring-atomic delivery, the point models and plain HH, and one update launch per step (not
+ * when epochs take the network over); otherwise nsx_step launches as before.  Results are
+ * bit-identical to per-step launches.  The barriers' waits are bounded: should one give up,
+ * nsx_step returns NSX_E_HIP and the state is not valid.  nsx_get_persistent reports the
+ * request and whether it applies to the loaded network now. */
+nsx_status nsx_set_persistent(nsx_ctx* ctx, int enable);
+nsx_status nsx_get_persistent(nsx_ctx* ctx, uint32_t* requested, uint32_t* applies);
+
 /* How propagated events reach their ring slot (the reference adds each event into
  * pending_input, engine.rs:414-494).  Results are bit-identical in both modes: the arrivals are
  * int64 sums, whatever their order.
@@ -395,9 +408,11 @@ typedef struct nsx_tune_report {
   uint32_t delivery, tile_entries;  /* chosen (tile entries: 0 atomic, 16384 pull)             */
   uint32_t trial_tile[48];          /* per trial: binned tile entries, else 0                  */
   uint32_t row_split, syn_block;    /* chosen row-split threshold (0: whole rows), workgroup  */
-  uint32_t update_width, _pad2;     /* chosen neurons per update thread (0: not a point model) */
+  uint32_t update_width, persistent; /* chosen neurons per update thread (0: not a point
+                                        model); chosen persistent stepping (0 / 1)           */
   uint32_t trial_row_split[48], trial_syn_block[48], trial_width[48];
   uint32_t trial_delivery[48];      /* per trial: NSX_DELIVERY_ATOMIC, _BINNED or _PULL         */
+  uint32_t trial_persistent[48];    /* per trial: 1 for persistent stepping                     */
 } nsx_tune_report;
 
 nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* report);

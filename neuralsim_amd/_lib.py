@@ -59,6 +59,8 @@ PROTOTYPES = {
     "nsx_set_row_split": (_int, [_VP, _u32]),
     "nsx_set_syn_block": (_int, [_VP, _u32]),
     "nsx_get_epoch": (_int, [_VP, C.POINTER(_u32)]),
+    "nsx_set_persistent": (_int, [_VP, _int]),
+    "nsx_get_persistent": (_int, [_VP, C.POINTER(_u32), C.POINTER(_u32)]),
     "nsx_set_delivery": (_int, [_VP, _u32, _u64, _u32]),
     "nsx_get_delivery": (_int, [_VP, C.POINTER(_u32)]),
     "nsx_read_raster": (_int, [_VP, _P, _P, _u64, C.POINTER(_u64), _int]),

@@ -1,7 +1,7 @@
 """ctypes mirror of include/nsx.h (types and constants only; loads nothing)."""
 import ctypes as C
 
-NSX_ABI_VERSION = 2
+NSX_ABI_VERSION = 3
 
 # nsx_status
 NSX_OK = 0
@@ -147,11 +147,12 @@ class NsxTuneReport(C.Structure):
         ("row_split", C.c_uint32),
         ("syn_block", C.c_uint32),
         ("update_width", C.c_uint32),
-        ("_pad2", C.c_uint32),
+        ("persistent", C.c_uint32),
         ("trial_row_split", C.c_uint32 * 48),
         ("trial_syn_block", C.c_uint32 * 48),
         ("trial_width", C.c_uint32 * 48),
         ("trial_delivery", C.c_uint32 * 48),
+        ("trial_persistent", C.c_uint32 * 48),
     ]
 
 

@@ -746,6 +746,24 @@ nsx_status nsx_set_syn_block(nsx_ctx* ctx, uint32_t threads) {
   return NSX_OK;
 }
 
+nsx_status nsx_set_persistent(nsx_ctx* ctx, int enable) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  ctx->persistent = enable != 0;
+  destroy_graph(ctx);
+  return NSX_OK;
+}
+
+nsx_status nsx_get_persistent(nsx_ctx* ctx, uint32_t* requested, uint32_t* applies) {
+  if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
+  if (applies && ctx->has_network) {
+    nsx_status s = epoch_prepare(ctx);  // whether epochs take the network over
+    if (s) return s;
+  }
+  if (requested) *requested = ctx->persistent ? 1u : 0u;
+  if (applies) *applies = ctx->has_network && persist_applies(ctx) ? 1u : 0u;
+  return NSX_OK;
+}
+
 nsx_status nsx_set_epoch(nsx_ctx* ctx, uint32_t max_steps) {
   if (!ctx) return set_error(nullptr, NSX_E_INVALID, "ctx is NULL");
   if (max_steps > 1024) return set_error(ctx, NSX_E_INVALID, "epoch longer than 1024 steps");
@@ -894,7 +912,23 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
       NSX_HIP(ctx, hipEventCreate(&ctx->ev_pool[k]));
   }
   uint64_t done = 0;
-  if (ctx->graph_steps > 0 && !prof) {
+  const bool persist = persist_applies(ctx) && !prof;
+  if (persist) {
+    // windows of graph_steps steps (1024 when graphs are off), one cooperative launch each
+    if (!ctx->d_bar) {
+      nsx_status sa = dev_alloc(ctx, (void**)&ctx->d_bar, 64, true);
+      if (sa) return sa;
+    }
+    const uint64_t W = ctx->graph_steps > 0 ? ctx->graph_steps : 1024;
+    while (done < n_steps) {
+      const uint32_t L = (uint32_t)std::min<uint64_t>(W, n_steps - done);
+      const int32_t off = (int32_t)(ctx->t - (uint64_t)ctx->tbase_host);
+      const hipError_t e = launch_persist(ctx, off, L);
+      if (e != hipSuccess) return hip_error(ctx, e, "persistent launch");
+      ctx->t += L;
+      done += L;
+    }
+  } else if (ctx->graph_steps > 0 && !prof) {
     const uint64_t G = ctx->graph_steps;
     if (n_steps >= G) {
       // graph offsets are relative to tbase: move tbase to the current step first
@@ -930,6 +964,14 @@ nsx_status nsx_step(nsx_ctx* ctx, uint64_t n_steps, nsx_step_stats* st) {
   }
   NSX_HIP(ctx, hipEventRecord(e_end, ctx->stream));
   NSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (persist) {
+    uint32_t gave_up = 0;
+    NSX_HIP(ctx, hipMemcpy(&gave_up, ctx->d_bar + 1, 4, hipMemcpyDeviceToHost));
+    if (gave_up)
+      return set_error(ctx, NSX_E_HIP,
+                       "persistent launch: a grid barrier gave up waiting; the state after this "
+                       "call is not valid");
+  }
   uint64_t c1[kNumPart];
   nsx_status s1 = read_totals(ctx, c1);
   if (s1) return s1;

@@ -295,6 +295,9 @@ struct nsx_ctx {
   uint64_t last_call_first_step = 0;
   // GpuBackend drop-in state
   bool backend_mode = false;
+  // persistent stepping: requested (nsx_set_persistent); the grid barrier's two words
+  bool persistent = false;
+  uint32_t* d_bar = nullptr;
   float* b_input = nullptr;     // uploaded input_current (f32)
   float* b_params = nullptr;    // 6 floats per neuron
   uint32_t* b_spiked = nullptr;
@@ -382,6 +385,10 @@ hipError_t enqueue_epoch(nsx_ctx* ctx, int32_t off, uint32_t len, hipEvent_t* ev
 hipError_t launch_epoch_surrogate(nsx_ctx* ctx, int32_t off, uint32_t len);
 // The loaded network fits binned delivery with tiles of tile_entries entries.
 bool bin_supported(const nsx_ctx* ctx, uint32_t tile_entries);
+// Persistent stepping (nsx_set_persistent): whether it applies to the context as configured
+// now, and one cooperative launch of `len` steps from *tbase + off.
+bool persist_applies(const nsx_ctx* ctx);
+hipError_t launch_persist(nsx_ctx* ctx, int32_t off, uint32_t len);
 }  // namespace nsx
 
 #define NSX_HIP(ctx, expr)                                          \

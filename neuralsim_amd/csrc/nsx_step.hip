@@ -29,13 +29,15 @@
 
 namespace nsx {
 
+// One step of neuron block blk (256 neurons, one per thread) at step t: the whole update of
+// k_update, shared with the persistent kernel (k_persist), which runs it for the blocks in turn.
 template <int MODEL, int DRIVE, int PL>
-__global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
+__device__ inline void update_block(const StepParams& p, const DevPtrs& s, int32_t t,
+                                    uint32_t blk) {
   constexpr bool TWO = MODEL != NSX_MODEL_IZHIKEVICH;
   __shared__ uint32_t s_drive[kBlock];
-  const int32_t t = *s.tbase + off;
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const NeuronId id = neuron_id(p, s, i, blockIdx.x);
+  const uint32_t i = blk * kBlock + threadIdx.x;
+  const NeuronId id = neuron_id(p, s, i, blk);
   const bool live = i < p.n && id.local < id.n;  // batch padding neurons stay idle
   // Every load of the neuron's update is issued here, ahead of the block's drive staging and
   // barrier, so that they are in flight together (one memory round trip, not one per field).
@@ -45,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
   int32_t refr = 0, last = kNeverStep;
   HHSyn y{0.0f, 0.0f, 0.0f, 0.0f};
   // blocks with no spike at t-1 (their segment count is 0) skip the last-spike loads
-  const uint32_t prev = PAIR ? s.bcnt[blockIdx.x] : 0u;
+  const uint32_t prev = PAIR ? s.bcnt[blk] : 0u;
   const bool ring_in = ring_active(p, s, t);
   if (live) {
     rg = ring_load<TWO>(p, s, i, t, ring_in);
@@ -153,8 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int3
     }
     if (spiked) spike_record<PL>(p, s, i, t);
   }
-  if (MODEL == NSX_MODEL_HH && p.stats_on) lfp_accumulate(s, lfp, blockIdx.x);
-  block_compact(p, s, spiked, i, t, blockIdx.x);
+  if (MODEL == NSX_MODEL_HH && p.stats_on) lfp_accumulate(s, lfp, blk);
+  block_compact(p, s, spiked, i, t, blk);
+}
+
+template <int MODEL, int DRIVE, int PL>
+__global__ __launch_bounds__(kBlock) void k_update(StepParams p, DevPtrs s, int32_t off) {
+  update_block<MODEL, DRIVE, PL>(p, s, *s.tbase + off, blockIdx.x);
 }
 
 // ---- LIF / Izhikevich update, four neurons per thread -------------------------------------
@@ -990,6 +997,85 @@ void k_synapses(StepParams p, DevPtrs s, int32_t off) {
   });
 }
 
+// ---- persistent stepping --------------------------------------------------------------------
+// nsx_set_persistent: a window of steps in one cooperative launch of a grid that is resident at
+// once.  Per step: the update of every neuron block (update_block, the workgroups taking the
+// blocks in turn), a grid barrier, the synapse pass with ring atomics (for_items over the same
+// work items as k_synapses), a grid barrier.  Every step's arithmetic and memory effects are
+// those of k_update and k_synapses, so the run is bit-identical to per-step launches.
+//
+// Grid barrier: bar[0] counts arrivals and only grows within a launch (zeroed before it), so
+// barrier k of a workgroup waits until bar[0] reaches (k + 1) x gridDim.x.  Every wavefront's
+// writes reach the device scope through its own release fence before the workgroup's arrival;
+// the acquire fences after the wait (every wavefront's, past the workgroup barrier) make the
+// other workgroups' writes visible.  The wait is bounded: past kBarrierSpins polls, or when
+// another workgroup has given up (bar[1] != 0), a workgroup sets bar[1] and leaves the kernel,
+// so every wavefront exits; the host then reports the failure.
+constexpr uint32_t kBarrierSpins = 1u << 22;
+
+__device__ inline bool grid_sync(uint32_t* bar, uint32_t& gen) {
+  __shared__ uint32_t s_ok;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t target = (gen + 1u) * gridDim.x;
+    atomicAdd(&bar[0], 1u);
+    uint32_t ok = 1u, spins = 0u;
+    while (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+          ++spins > kBarrierSpins) {
+        atomicOr(&bar[1], 1u);
+        ok = 0u;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(4);
+    }
+    s_ok = ok;
+  }
+  __syncthreads();
+  __threadfence();
+  gen += 1u;
+  return s_ok != 0u;
+}
+
+// The grid holds kPersistPerCu workgroups per CU: the compiler guarantees that many wavefronts
+// per SIMD (registers, scalar ones included, fit), so they are resident together.  The runtime's
+// occupancy query counts vector registers and LDS, not scalar registers, and can report more
+// workgroups per CU than fit: a grid sized by it alone waits at the first barrier for
+// workgroups that never start.
+constexpr uint32_t kPersistPerCu = 5;
+
+template <int MODEL, int DRIVE, int PL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kPersistPerCu)))
+void k_persist(StepParams p, DevPtrs s, int32_t off, uint32_t nsteps, uint32_t* bar) {
+  __shared__ RowChunk sh[kBlock / 64];
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  uint32_t gen = 0u;
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    const int32_t t = *s.tbase + off + (int32_t)k;
+    for (uint32_t b = blockIdx.x; b < p.nblk; b += gridDim.x) {
+      update_block<MODEL, DRIVE, PL>(p, s, t, b);
+      __syncthreads();  // the next block reuses the drive words' and wave counts' LDS
+    }
+    if (!grid_sync(bar, gen)) return;
+    for_items<PL, kBlock>(p, s, [&](bool prop, uint32_t b, uint32_t q, uint32_t cnt) {
+      if (prop) {
+        if (p.row_split)  // pieces of high-degree rows, as k_synapses<PL, NT, true>
+          propagate_item<PL, false, true>(p, s, sh[wid], t, b, q, cnt, lane, nullptr, nullptr);
+        else
+          propagate_item<PL, false, false>(p, s, sh[wid], t, b, q, cnt, lane, nullptr, nullptr);
+      } else if (PL != NSX_PLAST_NONE) {
+        depress_item<PL>(p, s, sh[wid], t, b, q, cnt, lane);
+      }
+    });
+    if (!grid_sync(bar, gen)) return;
+  }
+}
+
+__global__ void k_zero_bar(uint32_t* bar) {
+  if (threadIdx.x < 2) bar[threadIdx.x] = 0u;
+}
+
 // ---- epoch stepping -------------------------------------------------------------------------
 // Networks without plasticity whose shortest delay is d_min >= 2 steps: the spikes of steps
 // t .. t + d_min - 1 arrive at t + d_min at the earliest, so those steps' updates depend on no
@@ -1664,7 +1750,63 @@ static SynFn syn_kernel(int pl, uint32_t nt, bool pieces) {
   return pieces ? syn_kernel_pl<true>(pl, nt) : syn_kernel_pl<false>(pl, nt);
 }
 
-// Enqueue the launches of one step with offset `off` from *tbase.
+typedef void (*PersistFn)(StepParams, DevPtrs, int32_t, uint32_t, uint32_t*);
+
+template <int MODEL, int DRIVE>
+static PersistFn persist_plast(int pl) {
+  switch (pl) {
+    case NSX_PLAST_PAIR: return k_persist<MODEL, DRIVE, NSX_PLAST_PAIR>;
+    case NSX_PLAST_TRIPLET: return k_persist<MODEL, DRIVE, NSX_PLAST_TRIPLET>;
+    default: return k_persist<MODEL, DRIVE, NSX_PLAST_NONE>;
+  }
+}
+template <int MODEL>
+static PersistFn persist_drive(int drive, int pl) {
+  switch (drive) {
+    case NSX_DRIVE_POISSON: return persist_plast<MODEL, NSX_DRIVE_POISSON>(pl);
+    case NSX_DRIVE_GAUSSIAN: return persist_plast<MODEL, NSX_DRIVE_GAUSSIAN>(pl);
+    case NSX_DRIVE_OU: return persist_plast<MODEL, NSX_DRIVE_OU>(pl);
+    default: return persist_plast<MODEL, NSX_DRIVE_NONE>(pl);
+  }
+}
+static PersistFn pick_persist(int model, int drive, int pl) {
+  switch (model) {
+    case NSX_MODEL_LIF: return persist_drive<NSX_MODEL_LIF>(drive, pl);
+    case NSX_MODEL_IZHIKEVICH: return persist_drive<NSX_MODEL_IZHIKEVICH>(drive, pl);
+    default: return persist_drive<NSX_MODEL_HH>(drive, pl);
+  }
+}
+
+// Whether the loaded network can step persistently: a plain update model (not the surrogate,
+// not the GpuBackend mode), ring-atomic delivery, one update launch per step (no epochs).
+bool persist_applies(const nsx_ctx* ctx) {
+  return ctx->persistent && ctx->cfg.model != NSX_MODEL_HH_SURROGATE && !ctx->backend_mode &&
+         ctx->delivery == NSX_DELIVERY_ATOMIC && !ctx->p.bin_on && !ctx->p.pull_on &&
+         epoch_len(ctx) == 1;
+}
+
+// One cooperative launch of `len` steps from offset `off`; the grid is kPersistPerCu
+// workgroups per CU, fewer if the runtime's occupancy query reports fewer (the launch fails
+// when the grid cannot be resident at once, rather than deadlock).
+hipError_t launch_persist(nsx_ctx* ctx, int32_t off, uint32_t len) {
+  const PersistFn f = pick_persist((int)ctx->cfg.model, (int)ctx->cfg.drive,
+                                   (int)ctx->cfg.plasticity);
+  int per_cu = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)f, kBlock, 0);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  if (e != hipSuccess) return e;
+  const uint32_t grid = (uint32_t)std::min(std::max(per_cu, 1), (int)kPersistPerCu) *
+                       (uint32_t)std::max(cus, 1);
+  hipLaunchKernelGGL(k_zero_bar, dim3(1), dim3(64), 0, ctx->stream, ctx->d_bar);
+  StepParams p = ctx->p;
+  DevPtrs d = ctx->d;
+  uint32_t* bar = ctx->d_bar;
+  void* args[] = {&p, &d, &off, &len, &bar};
+  return hipLaunchCooperativeKernel((const void*)f, dim3(grid), dim3(kBlock), args, 0,
+                                    ctx->stream);
+}
+
 // Arrival of the bucketed events at step *tbase + off (emission steps up to emax).
 hipError_t launch_arrive(nsx_ctx* ctx, int32_t off, int32_t emax) {
   const StepParams& p = ctx->p;
@@ -1677,8 +1819,9 @@ hipError_t launch_arrive(nsx_ctx* ctx, int32_t off, int32_t emax) {
   return hipGetLastError();
 }
 
-// Enqueue the launches of one step with offset `off` from *tbase.  Profiling events:
-// [0] update [1] binned counting passes [2] synapse pass [3] binned arrival [4].
+// Enqueue the launches of one step with offset `off` from *tbase.  Profiling events: binned
+// delivery [0] update [1] synapse pass [2] bucket passes [3] arrival [4]; otherwise [0] update
+// [1] [2] synapse pass (or pull) [3] [4].
 hipError_t enqueue_step(nsx_ctx* ctx, int32_t off, hipEvent_t* evs) {
   const StepParams& p = ctx->p;
   const int pl = (int)ctx->cfg.plasticity;

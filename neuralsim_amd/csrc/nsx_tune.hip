@@ -4,8 +4,8 @@
 // step's cost depends on beyond the network itself: the propagation split (wavefronts that
 // share one neuron block's spike segment), the row-split threshold that cuts high-degree rows
 // into pieces those wavefronts share, the synapse pass's workgroup size, the point models'
-// neurons per update thread, the graph window (steps per captured hipGraph launch) and the
-// delivery (ring atomics or binned, with its tile size).  The neuron block of the spike
+// neurons per update thread, the graph window (steps per captured hipGraph launch), the
+// delivery (ring atomics or binned, with its tile size) and persistent stepping.  The neuron block of the spike
 // segments stays 256 neurons.  Every trial times the same steps from a
 // device-side snapshot of the mutable state (after its warm-up / capture the state is restored)
 // and the state is restored afterwards, so tuning does not change the simulation.
@@ -122,6 +122,7 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   const uint32_t deliv0 = ctx->delivery, te0 = ctx->bin_te_req;
   const uint32_t rs0 = ctx->p.row_split, nt0 = ctx->syn_block, width0 = ctx->update_width;
   const uint64_t cap0 = ctx->bin_cap_req;
+  const bool persist0 = ctx->persistent;
   const uint32_t stats0 = ctx->p.stats_on;  // trials neither count nor fold statistics
   const bool point = ctx->cfg.model == NSX_MODEL_LIF || ctx->cfg.model == NSX_MODEL_IZHIKEVICH;
   const bool per_step = epoch_len(ctx) == 1;  // an epoch launch has its own update and block
@@ -132,9 +133,10 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   double best = 1e300;
   struct Shape {
     uint32_t split, graph, dl, te, rs, nt, width;  // dl: delivery; te: binned tile entries
+    uint32_t ps;                                   // persistent stepping
   };
   Shape cur{split0, graph0 ? graph0 : 50, deliv0,
-            deliv0 == NSX_DELIVERY_BINNED ? ctx->bin_te : 0u, rs0, nt0, width0};
+            deliv0 == NSX_DELIVERY_BINNED ? ctx->bin_te : 0u, rs0, nt0, width0, 0u};
   Shape bs = cur;
   auto trial = [&](const Shape& sh) -> nsx_status {
     nsx_status st =
@@ -144,6 +146,7 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     if (!st) st = nsx_set_syn_block(ctx, sh.nt);
     if (!st) st = nsx_set_update_width(ctx, sh.width);
     if (!st) st = nsx_set_graph_steps(ctx, sh.graph);
+    if (!st) st = nsx_set_persistent(ctx, (int)sh.ps);
     if (st) return st;
     // the same number of steps in every trial: a multiple of every window tried (200)
     const uint64_t steps = (trial_steps + 199) / 200 * 200;
@@ -167,6 +170,7 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
       r.trial_row_split[k] = sh.rs;
       r.trial_syn_block[k] = sh.nt;
       r.trial_width[k] = sh.width;
+      r.trial_persistent[k] = sh.ps;
       r.us_per_step[k] = us;
     }
     if (us < best) {
@@ -250,6 +254,30 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
       if (!s && sp != base.split) s = trial(sh);
     }
   }
+  // 6. persistent stepping (ring atomics, one update launch per step), windows of the best
+  // graph window and of 200 steps.  A device that cannot hold the grid at once refuses the
+  // cooperative launch: that trial is dropped and the state restored.
+  if (!s && bs.dl == NSX_DELIVERY_ATOMIC && per_step && !ctx->backend_mode &&
+      ctx->cfg.model != NSX_MODEL_HH_SURROGATE) {
+    const Shape base = bs;
+    for (uint32_t g : {base.graph, 200u}) {
+      Shape sh = base;
+      sh.ps = 1u;
+      sh.graph = g;
+      if (g == 200u && base.graph == 200u) continue;
+      if (trial(sh) != NSX_OK) {
+        (void)hipGetLastError();
+        hipError_t e = copy_all(ctx, bufs, snap, false);
+        if (e != hipSuccess) {
+          s = hip_error(ctx, e, "autotune restore");
+          break;
+        }
+        ctx->t = t0;
+        ctx->tbase_host = tb0;
+        break;
+      }
+    }
+  }
   dev_free(ctx, snap);
   ctx->profiling = prof;
   ctx->p.stats_on = stats0;
@@ -261,13 +289,14 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
     (void)nsx_set_syn_block(ctx, nt0);
     (void)nsx_set_update_width(ctx, width0);
     (void)nsx_set_graph_steps(ctx, graph0);
+    (void)nsx_set_persistent(ctx, persist0 ? 1 : 0);
     return s;
   }
   if ((s = bin_configure(ctx, bs.dl, cap0, bs.dl == NSX_DELIVERY_BINNED ? bs.te : 0u)))
     return s;
   if ((s = nsx_set_prop_split(ctx, bs.split)) || (s = nsx_set_row_split(ctx, bs.rs)) ||
       (s = nsx_set_syn_block(ctx, bs.nt)) || (s = nsx_set_update_width(ctx, bs.width)) ||
-      (s = nsx_set_graph_steps(ctx, bs.graph)))
+      (s = nsx_set_graph_steps(ctx, bs.graph)) || (s = nsx_set_persistent(ctx, (int)bs.ps)))
     return s;
   r.prop_split = bs.split;
   r.graph_steps = bs.graph;
@@ -276,6 +305,7 @@ nsx_status nsx_autotune(nsx_ctx* ctx, uint32_t trial_steps, nsx_tune_report* rep
   r.row_split = bs.rs;
   r.syn_block = bs.nt;
   r.update_width = point ? bs.width : 0u;
+  r.persistent = bs.ps;
   r.best_us_per_step = best;
   if (rep) *rep = r;
   return NSX_OK;

@@ -188,6 +188,19 @@ class SimulationEngine:
         self._chk(self._lib.nsx_set_update_width(self._h, int(neurons_per_thread)),
                   "nsx_set_update_width")
 
+    def set_persistent(self, enable=True):
+        """Step windows of steps as one cooperative launch each (ring-atomic delivery, one update
+        launch per step); results are unchanged."""
+        self._chk(self._lib.nsx_set_persistent(self._h, 1 if enable else 0), "nsx_set_persistent")
+
+    @property
+    def persistent(self):
+        """(requested, applies to the loaded network now)."""
+        r, a = C.c_uint32(), C.c_uint32()
+        self._chk(self._lib.nsx_get_persistent(self._h, C.byref(r), C.byref(a)),
+                  "nsx_get_persistent")
+        return bool(r.value), bool(a.value)
+
     def set_epoch(self, max_steps):
         """Steps per epoch at most (<= 1: one step per launch)."""
         self._chk(self._lib.nsx_set_epoch(self._h, max_steps), "nsx_set_epoch")
@@ -231,7 +244,8 @@ class SimulationEngine:
         return [{"split": r.trial_split[k], "graph": r.trial_graph[k],
                  "delivery": r.trial_delivery[k], "tile": r.trial_tile[k],
                  "row_split": r.trial_row_split[k], "syn_block": r.trial_syn_block[k],
-                 "width": r.trial_width[k], "us_per_step": r.us_per_step[k]}
+                 "width": r.trial_width[k], "persistent": r.trial_persistent[k],
+                 "us_per_step": r.us_per_step[k]}
                 for k in range(r.n_trials)]
 
     def set_row_split(self, slots):

@@ -1,7 +1,8 @@
 """The autotuner measures launch shapes from a snapshot and restores it: a run that is tuned
 midway is bit-identical to an untuned run (raster, state, STDP weights).  The shapes it chooses
 among -- row-split threshold for high-degree rows, synapse workgroup size, propagation split --
-leave every result bit-identical, under pair and triplet STDP and in epoch stepping."""
+leave every result bit-identical, under pair and triplet STDP and in epoch stepping.  Persistent
+stepping is among the trials when ring atomics win."""
 import numpy as np
 import pytest
 
@@ -18,9 +19,13 @@ def test_autotune_leaves_simulation_unchanged(nsx):
         split, graph, trials = a.autotune(40)
         print("chosen", split, graph, trials)
         # 5 splits; row-split thresholds, workgroup sizes and update widths under atomic
-        # delivery; 4 windows; the two binned tile sizes; the split again if binned won
+        # delivery; 4 windows; the two binned tile sizes; the split again if binned won;
+        # persistent stepping (one or two windows) if ring atomics won
         tt = a.tune_trials()
-        assert len(trials) in (18, 22) and split in (1, 2, 4, 8, 16)
+        n_ps = sum(t["persistent"] for t in tt)
+        assert len(trials) - n_ps in (18, 22) and split in (1, 2, 4, 8, 16)
+        assert n_ps in (1, 2) if a.delivery == 0 else n_ps == 0
+        assert a.persistent[0] == bool(a.last_tune.persistent)
         assert graph in (10, 25, 50, 100, 200)
         assert all(t[2] > 0 for t in trials)
         assert {t["row_split"] for t in tt} >= {0, 256, 128, 64}

@@ -97,6 +97,8 @@ def test_null_arguments_return_status(nsx):
     assert lib.nsx_backend_new(0, 0, None) == abi.NSX_E_INVALID
     n = C.c_int(-1)
     assert lib.nsx_device_count(None) == abi.NSX_E_INVALID
+    assert lib.nsx_set_persistent(None, 1) == abi.NSX_E_INVALID
+    assert lib.nsx_get_persistent(None, None, None) == abi.NSX_E_INVALID
     lib.nsx_destroy(None)  # no-op
 
 

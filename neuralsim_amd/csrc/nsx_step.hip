@@ -560,16 +560,22 @@ constexpr uint32_t kBinEntryBits = 14;  // tile entries <= 16384
 static_assert(kMaxBins << kBinEntryBits <= (1u << 27), "bucket and entry share one word");
 
 // The slots of one row chunk (chunk_setup done): propagation and the STDP at the pre spike.
-// Each lane takes kPropSlots slots per round, 64 apart, and issues all their loads before the
-// first atomic.  A slot past the chunk's end repeats the previous slot's loads (in bounds) and
-// stores nothing.  Returns the potentiated / depressed-at-pre slot count of this lane.
+// Each lane takes NS slots per round, 64 apart, and issues all their loads before the first
+// atomic or record store.  A slot past the chunk's end repeats the previous slot's loads (in
+// bounds) and stores nothing.  Returns the potentiated / depressed-at-pre slot count of this
+// lane.  NS: kPropSlots in the ring-atomic kernels (their occupancy bound); the binned pass's
+// 1024-thread workgroups run at 4 waves per SIMD anyway, so it takes more slots in flight --
+// measured: 4 under STDP (1M network 133.1 -> 130.8 us per step; 6 spills registers and ran
+// 132.9), 6 without plasticity (population 0.1796 -> 0.1741 ms per step; 4: 0.1755, 8: 0.1740).
 constexpr uint32_t kPropSlots = 2;
+constexpr uint32_t kBinSlotsPlastic = 4, kBinSlotsPlain = 6;
 
 template <int PL, bool BIN>
 __device__ unsigned long long propagate_chunk(const StepParams& p, const DevPtrs& s,
                                               const RowChunk& rc, int32_t t, uint32_t total,
                                               uint32_t lane, uint32_t* pos, uint2* bev) {
-  constexpr uint32_t NS = kPropSlots;
+  constexpr uint32_t NS =
+      !BIN ? kPropSlots : PL == NSX_PLAST_NONE ? kBinSlotsPlain : kBinSlotsPlastic;
   const uint32_t tm = (uint32_t)t % p.D;
   const size_t plane = (size_t)p.ch * p.n;
   unsigned long long ltp = 0;

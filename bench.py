@@ -619,7 +619,7 @@ def main():
     ap.add_argument("--burnin-ms", type=float, default=1000.0,
                     help="simulated ms run before the warm-up (out of the quiet initial state)")
     ap.add_argument("--neurons", type=int, default=1_000_000)
-    ap.add_argument("--graph", type=int, default=50, help="steps per captured hipGraph (0: off)")
+    ap.add_argument("--graph", type=int, default=100, help="steps per captured hipGraph (0: off)")
     ap.add_argument("--no-profile", action="store_true", help="no per-kernel replay")
     ap.add_argument("--split", type=int, default=0, help="wavefronts per neuron block (0: default)")
     ap.add_argument("--delivery", choices=("binned", "atomic", "auto"), default="binned",
